@@ -1,0 +1,146 @@
+#!/usr/bin/env python3
+"""Benchmark of the gfx950 SPH step (one MI355X).
+
+Default workload: uniform_1m -- 1,048,576 particles uniform in the lower half
+of a 16x8x16 box, spatial-hash table 2^21, h = 0.2, dt = 1/120 in 3 substeps.
+A step is one sph_step(1/120, 3): three Update(dt/3) calls of the reference
+(physicsWorld.cc:39-111), i.e. three particle-substeps per particle.
+
+Output: one JSON line (rank 0).  `value` is particle-steps/s with the state
+resident in HBM when the timed region starts.  `roofline` is computed for the
+dominant kernel from its mean duration, measured live over the timed region
+with HIP events recorded on the simulation's own stream.  `cpu_baseline` times
+the CPU oracle (a restatement of the reference step, the reference's `par`
+loops run serially as they do without TBB) on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "fluid-simulation-3d_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import sph_amd as S  # noqa: E402
+from sph_amd import workloads as W  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic HBM bytes per particle per launch of each timed stage: each
+# array read or written once, neighbour re-reads counted as on-chip.
+#   predict:   R pos, vel (16+16), W vel, xp (16+16), W key, slot (4+4)
+#   density:   R xp (16), W rho, rho_n, P, P_n (16); bucket table reads ~0
+#   pressure:  R xp, dens, v (48), W v (16)
+#   viscosity+integrate: R xp, v, slot (36), R+W pos (32), W vel (16)
+ALG_BYTES = {"predict": 72.0, "density": 32.0, "pressure": 64.0, "viscosity_integrate": 84.0}
+STAGES = ["predict", "spatial", "density", "pressure", "viscosity_integrate"]
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(sc, substeps):
+    """Time the oracle (single thread) on `substeps` Update calls of the same input."""
+    import pyoracle as O
+    O.set_threads(1)
+    dt = float(np.float32(np.float32(sc.dt) / np.float32(sc.substeps)))
+    op = O.from_sph(sc.params)
+    t0 = time.perf_counter()
+    O.update(op, dt, sc.pos, sc.vel, T=sc.table, substeps=substeps)
+    el = time.perf_counter() - t0
+    psub = sc.n * substeps / el
+    return {"value": psub / sc.substeps, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{substeps} Update(dt/3) calls of {sc.name} (n={sc.n}, table={sc.table}) "
+                      f"on 1 host thread, {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="uniform_1m")
+    ap.add_argument("--latency-steps", type=int, default=100)
+    ap.add_argument("--cpu-substeps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(every box has one MI355X; sharding is described in DESIGN.md)", file=sys.stderr)
+        return 2
+    if S.device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        return 1
+
+    sc = W.by_name(args.workload)
+    log(f"[bench] workload {sc.name} n={sc.n} table={sc.table} steps={args.steps} warmup={args.warmup}")
+    sim = S.Sim(sc.params, sc.n, sc.table)
+    sim.upload(sc.pos, sc.vel)
+    for _ in range(args.warmup):
+        sim.step_async(sc.dt, sc.substeps)
+    sim.sync()
+    log("[bench] warmup done")
+
+    # timed region: K steps, stage events recorded on the simulation stream
+    sim.profile_begin(args.steps * sc.substeps)
+    sim.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sim.step_async(sc.dt, sc.substeps)
+    sim.sync()
+    elapsed = time.perf_counter() - t0
+    stage_ms, n_upd = sim.profile_end()
+    log(f"[bench] timed {args.steps} steps in {elapsed:.3f} s; stage ms {np.round(stage_ms[:5], 4)}")
+
+    # per-step latency (blocking steps, graph replay), outside the timed region
+    lat = []
+    for _ in range(args.latency_steps):
+        t = time.perf_counter()
+        sim.step(sc.dt, sc.substeps)
+        lat.append((time.perf_counter() - t) * 1e3)
+    lat = np.array(lat)
+    nonfinite = sim.count_nonfinite()
+    sim.close()
+
+    value = sc.n * args.steps / elapsed
+    per = {STAGES[i]: float(stage_ms[i]) for i in range(5)}
+    gather = ["density", "pressure", "viscosity_integrate"]
+    dom = max(gather, key=lambda k: per[k])
+    ach = ALG_BYTES[dom] * sc.n / (per[dom] * 1e-3) / 1e9
+    out = {
+        "metric": "particle-steps/s (single scene, 1 MI355X)",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": sc.name, "particles": sc.n, "table_size": sc.table,
+                   "h": sc.params.smoothing_radius, "dt": sc.dt, "substeps": sc.substeps,
+                   "parallelism": "single GPU"},
+        "step_latency_ms": {"p50": float(np.percentile(lat, 50)),
+                            "p99": float(np.percentile(lat, 99)), "n": int(lat.size)},
+        "stage_ms_per_update": per,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                     "alg_bytes_per_particle": ALG_BYTES[dom]},
+        "nonfinite_components": nonfinite,
+    }
+    if not args.no_cpu:
+        log("[bench] cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(sc, args.cpu_substeps)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

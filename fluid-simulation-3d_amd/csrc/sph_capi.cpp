@@ -1,0 +1,288 @@
+// sph_capi.cpp -- the C ABI of include/sph_amd.h over sph::Engine.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/sph_amd.h"
+#include "sph_device.h"
+#include "sph_engine.h"
+
+struct sph_sim {
+    sph::Engine *eng;
+};
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+int done(int rc, const std::string &msg)
+{
+    if (rc != SPH_OK) g_err = msg;
+    return rc;
+}
+
+bool have_gpu()
+{
+    int n = 0;
+    return hipGetDeviceCount(&n) == hipSuccess && n > 0;
+}
+}  // namespace
+
+#define SPH_GUARD_BEGIN try {
+#define SPH_GUARD_END                                                     \
+    }                                                                     \
+    catch (const std::bad_alloc &) { return fail(SPH_ERR_OOM, "host allocation failed"); } \
+    catch (const std::exception &ex) { return fail(SPH_ERR_INVALID, ex.what()); }          \
+    catch (...) { return fail(SPH_ERR_INVALID, "unknown exception"); }
+
+#define SPH_NEED(h)                                                       \
+    if (!(h) || !(h)->eng) return fail(SPH_ERR_INVALID, "null handle")
+
+extern "C" {
+
+void sph_default_params(sph_params *p)
+{
+    if (!p) return;
+    p->smoothing_radius = 0.35f;       // physicsWorld.h:97
+    p->target_density = 99.7f;         // physicsWorld.h:98
+    p->pressure_mult = 300.0f;         // physicsWorld.h:99
+    p->near_pressure_mult = 20.0f;     // physicsWorld.h:100
+    p->viscosity = 0.5f;               // physicsWorld.h:101
+    p->gravity_on = 0;                 // physicsWorld.h:105
+    p->gravity_scale = 10.0f;          // physicsWorld.h:106
+    p->bounds[0] = p->bounds[1] = p->bounds[2] = 20.0f;  // physicsWorld.h:145
+    p->lookahead = 1.0f / 120.0f;      // physicsWorld.cc:47
+    p->damp = 0.95f;                   // physicsWorld.cc:87
+    p->prefilter_r2 = 0.35f * 0.35f;   // physicsWorld.h:96
+}
+
+const char *sph_last_error(void) { return g_err.c_str(); }
+
+const char *sph_version(void) { return "sph_amd 0.1 (gfx950)"; }
+
+uint32_t sph_hash_cell(int32_t cx, int32_t cy, int32_t cz) { return sph::hash_cell(cx, cy, cz); }
+
+uint32_t sph_bucket_of(uint32_t hash, uint32_t table_size)
+{
+    if (table_size == 0) return 0;
+    return sph::fastmod(hash, sph::fastmod_constant(table_size), table_size);
+}
+
+int sph_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int sph_create(const sph_params *p, uint32_t n, uint32_t table_size, sph_handle *out)
+{
+    SPH_GUARD_BEGIN
+    if (!p || !out) return fail(SPH_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (n == 0) return fail(SPH_ERR_INVALID, "n must be > 0");
+    if (!have_gpu()) return fail(SPH_ERR_HIP, "no HIP device visible");
+    sph::SceneSpec spec{*p, n, table_size ? table_size : n};
+    std::string err;
+    sph::Engine *e = nullptr;
+    int rc = sph::Engine::create({spec}, n, spec.table, 1, &e, &err);
+    if (rc) return fail(rc, err);
+    *out = new sph_sim{e};
+    return SPH_OK;
+    SPH_GUARD_END
+}
+
+int sph_destroy(sph_handle h)
+{
+    SPH_GUARD_BEGIN
+    if (!h) return SPH_OK;
+    delete h->eng;
+    delete h;
+    return SPH_OK;
+    SPH_GUARD_END
+}
+
+int sph_set_params(sph_handle h, const sph_params *p)
+{
+    SPH_GUARD_BEGIN
+    SPH_NEED(h);
+    if (!p) return fail(SPH_ERR_INVALID, "null params");
+    std::string err;
+    return done(h->eng->set_params(0, *p, &err), err);
+    SPH_GUARD_END
+}
+
+int sph_get_params(sph_handle h, sph_params *p)
+{
+    SPH_NEED(h);
+    if (!p) return fail(SPH_ERR_INVALID, "null params");
+    *p = h->eng->params(0);
+    return SPH_OK;
+}
+
+int sph_get_size(sph_handle h, uint32_t *n, uint32_t *table_size)
+{
+    SPH_NEED(h);
+    if (n) *n = h->eng->scene(0).n;
+    if (table_size) *table_size = h->eng->scene(0).table;
+    return SPH_OK;
+}
+
+int sph_upload(sph_handle h, const float *pos4, const float *vel4)
+{
+    SPH_GUARD_BEGIN
+    SPH_NEED(h);
+    std::string err;
+    return done(h->eng->upload(0, pos4, vel4, &err), err);
+    SPH_GUARD_END
+}
+
+int sph_initialize_data(sph_handle h)
+{
+    SPH_GUARD_BEGIN
+    SPH_NEED(h);
+    const uint32_t n = h->eng->scene(0).n;
+    std::vector<float> pos(4 * (size_t)n), vel(4 * (size_t)n, 0.0f);
+    const float box[3] = {0, 0, 0};
+    int rc = sph_generate(SPH_INIT_GRID_ARRANGEMENT, 0, n, box, 0.0f, 0.0f, 0.0f, pos.data(),
+                          vel.data());
+    if (rc) return rc;
+    std::string err;
+    rc = h->eng->upload(0, pos.data(), vel.data(), &err);
+    if (!rc) rc = h->eng->prime(&err);
+    return done(rc, err);
+    SPH_GUARD_END
+}
+
+int sph_update(sph_handle h, float dt)
+{
+    SPH_GUARD_BEGIN
+    SPH_NEED(h);
+    std::string err;
+    int rc = h->eng->update(dt, &err);
+    if (!rc) rc = h->eng->sync(&err);
+    return done(rc, err);
+    SPH_GUARD_END
+}
+
+int sph_step(sph_handle h, float dt, int substeps)
+{
+    SPH_GUARD_BEGIN
+    SPH_NEED(h);
+    std::string err;
+    return done(h->eng->step(dt, substeps, true, &err), err);
+    SPH_GUARD_END
+}
+
+int sph_step_async(sph_handle h, float dt, int substeps)
+{
+    SPH_GUARD_BEGIN
+    SPH_NEED(h);
+    std::string err;
+    return done(h->eng->step(dt, substeps, false, &err), err);
+    SPH_GUARD_END
+}
+
+int sph_sync(sph_handle h)
+{
+    SPH_GUARD_BEGIN
+    SPH_NEED(h);
+    std::string err;
+    return done(h->eng->sync(&err), err);
+    SPH_GUARD_END
+}
+
+int sph_download(sph_handle h, float *pos4, float *vel4, float *dens2)
+{
+    SPH_GUARD_BEGIN
+    SPH_NEED(h);
+    std::string err;
+    return done(h->eng->download(0, pos4, vel4, dens2, &err), err);
+    SPH_GUARD_END
+}
+
+int sph_set_timing(sph_handle h, int on)
+{
+    SPH_NEED(h);
+    h->eng->set_timing(on != 0);
+    return SPH_OK;
+}
+
+int sph_stage_ms(sph_handle h, double ms[6])
+{
+    SPH_NEED(h);
+    if (!ms) return fail(SPH_ERR_INVALID, "null output");
+    memcpy(ms, h->eng->stage_ms(), sizeof(double) * 6);
+    return SPH_OK;
+}
+
+int sph_profile_begin(sph_handle h, uint32_t max_updates)
+{
+    SPH_GUARD_BEGIN
+    SPH_NEED(h);
+    std::string err;
+    return done(h->eng->profile_begin(max_updates, &err), err);
+    SPH_GUARD_END
+}
+
+int sph_profile_end(sph_handle h, double mean_ms[6], uint32_t *updates)
+{
+    SPH_GUARD_BEGIN
+    SPH_NEED(h);
+    if (!mean_ms) return fail(SPH_ERR_INVALID, "null output");
+    std::string err;
+    return done(h->eng->profile_end(mean_ms, updates, &err), err);
+    SPH_GUARD_END
+}
+
+int sph_count_nonfinite(sph_handle h, uint64_t *count)
+{
+    SPH_GUARD_BEGIN
+    SPH_NEED(h);
+    if (!count) return fail(SPH_ERR_INVALID, "null output");
+    std::string err;
+    return done(h->eng->count_nonfinite(count, &err), err);
+    SPH_GUARD_END
+}
+
+int sph_debug_spatial(sph_handle h, uint32_t *hash_id, uint32_t *key_id, uint32_t *sorted_key,
+                      uint32_t *sorted_idx, uint32_t *start, uint32_t *end)
+{
+    SPH_GUARD_BEGIN
+    SPH_NEED(h);
+    std::string err;
+    return done(h->eng->debug_spatial(0, hash_id, key_id, sorted_key, sorted_idx, start, end,
+                                      &err),
+                err);
+    SPH_GUARD_END
+}
+
+int sph_debug_predicted(sph_handle h, float *xp4)
+{
+    SPH_GUARD_BEGIN
+    SPH_NEED(h);
+    if (!xp4) return fail(SPH_ERR_INVALID, "null output");
+    std::string err;
+    return done(h->eng->debug_predicted(0, xp4, &err), err);
+    SPH_GUARD_END
+}
+
+int sph_debug_neighbours(sph_handle h, uint64_t *offs, uint32_t *list, uint64_t cap,
+                         uint64_t *total)
+{
+    SPH_GUARD_BEGIN
+    SPH_NEED(h);
+    std::string err;
+    return done(h->eng->debug_neighbours(0, offs, list, cap, total, &err), err);
+    SPH_GUARD_END
+}
+
+}  // extern "C"

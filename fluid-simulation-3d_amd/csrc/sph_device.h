@@ -1,0 +1,44 @@
+// sph_device.h -- integer and floating-point building blocks shared by the
+// gfx950 kernels and by host code (the same source compiles for both, so the
+// host-side unit tests exercise the exact device arithmetic).
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define SPH_HD __host__ __device__ __forceinline__
+#else
+#define SPH_HD inline
+#endif
+
+namespace sph {
+
+// HashCell (physicsWorld.cc:505-511) on integer cell coordinates.  The
+// reference converts the float cell coordinate to uint32_t; on its x86-64
+// builds a negative value wraps (two's complement), which the int32 -> uint32
+// conversion here reproduces for |cell| < 2^31.  (AMDGPU's direct
+// float -> u32 conversion would saturate negatives to 0 instead.)
+SPH_HD uint32_t hash_cell(int32_t cx, int32_t cy, int32_t cz)
+{
+    return (uint32_t)cx * 15823u + (uint32_t)cy * 9737333u + (uint32_t)cz * 440817757u;
+}
+
+// hash % table (GetKeyFromHash, physicsWorld.cc:513-516) without a divide:
+// Lemire, Kaser & Kurz, "Faster remainder by direct computation" (2019).
+// m = floor((2^64 - 1) / table) + 1 (m = 0 for table == 1, still exact).
+// Exact for every 32-bit numerator and divisor.
+SPH_HD uint64_t fastmod_constant(uint32_t table) { return ~0ull / table + 1ull; }
+
+SPH_HD uint32_t fastmod(uint32_t a, uint64_t m, uint32_t table)
+{
+    const uint64_t low = m * (uint64_t)a;  // mod 2^64
+    const uint32_t lo = (uint32_t)low, hi = (uint32_t)(low >> 32);
+    // floor(low * table / 2^64) = floor((hi*table + floor(lo*table / 2^32)) / 2^32)
+    const uint64_t lo_hi = ((uint64_t)lo * table) >> 32;
+    return (uint32_t)(((uint64_t)hi * table + lo_hi) >> 32);
+}
+
+// glm::sign for float (func_common.inl:144-150): (0 < x) - (x < 0).
+SPH_HD float sign_f(float x) { return (float)(0.0f < x) - (float)(x < 0.0f); }
+
+}  // namespace sph

@@ -1,0 +1,622 @@
+// sph_engine.cpp -- host side of the gfx950 SPH step (see sph_engine.h).
+#include "sph_engine.h"
+
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+
+#include "sph_device.h"
+
+namespace sph {
+
+namespace {
+// glm::pi<float>() (exts/glm/ext/scalar_constants.inl:13-17)
+const float kPiF = (float)3.14159265358979323846264338327950288;
+
+uint32_t bits_for(uint32_t max_value)
+{
+    uint32_t b = 1;
+    while (b < 32 && (max_value >> b) != 0) ++b;
+    return b;
+}
+
+bool finite_f(float x) { return isfinite(x) != 0; }
+}  // namespace
+
+float sqrt_threshold(float h)
+{
+    if (!(h > 0.0f)) return 0.0f;
+    if (isinf(h)) return INFINITY;
+    // sqrtf is correctly rounded, hence monotone: binary search the first
+    // non-negative float whose root reaches h.
+    uint32_t lo = 0, hi = 0x7f800000u;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        float x;
+        memcpy(&x, &mid, 4);
+        if (sqrtf(x) >= h) hi = mid;
+        else lo = mid + 1;
+    }
+    float r;
+    memcpy(&r, &lo, 4);
+    return r;
+}
+
+const char *check_params(const sph_params &p)
+{
+    if (!finite_f(p.smoothing_radius) || !(p.smoothing_radius > 0.0f))
+        return "smoothing_radius must be finite and > 0";
+    if (!finite_f(p.target_density) || !finite_f(p.pressure_mult) ||
+        !finite_f(p.near_pressure_mult) || !finite_f(p.viscosity) || !finite_f(p.gravity_scale))
+        return "settings must be finite";
+    for (int c = 0; c < 3; ++c)
+        if (!finite_f(p.bounds[c]) || !(p.bounds[c] > 0.0f)) return "bounds must be finite and > 0";
+    if (!finite_f(p.lookahead) || !finite_f(p.damp)) return "lookahead and damp must be finite";
+    if (!finite_f(p.prefilter_r2) || p.prefilter_r2 < 0.0f) return "prefilter_r2 must be finite and >= 0";
+    return nullptr;
+}
+
+void fill_scene_desc(const sph_params &p, uint32_t n, uint32_t table, SceneDesc *d)
+{
+    const float r = p.smoothing_radius;
+    d->n = n;
+    d->table = table;
+    d->mod_m = fastmod_constant(table);
+    d->h = r;
+    d->r2 = p.prefilter_r2;
+    d->h2_lt = sqrt_threshold(r);
+    d->rho0 = p.target_density;
+    d->k = p.pressure_mult;
+    d->k_near = p.near_pressure_mult;
+    d->mu = p.viscosity;
+    d->w2c = 15 / (2 * kPiF * powf(r, 5));               // kernels.h:29
+    d->w3c = 15.0f / (kPiF * powf(r, 6));                // kernels.h:41
+    d->dw2c = 15.0f / (powf(r, 5) * kPiF);               // kernels.h:53
+    d->dw3c = 45 / (powf(r, 6) * kPiF);                  // kernels.h:65
+    d->poly6c = 315 / (64 * kPiF * powf(fabsf(r), 9));   // kernels.h:77
+    d->hh = r * r;                                       // kernels.h:78
+    d->g[0] = 0.0f;                                      // cc:315-320
+    d->g[1] = p.gravity_on ? -p.gravity_scale : 0.0f;
+    d->g[2] = 0.0f;
+    d->lookahead = p.lookahead;                          // cc:47
+    for (int c = 0; c < 3; ++c) d->half[c] = p.bounds[c] * 0.5f;  // cc:88
+    d->damp_neg = -1 * p.damp;                           // cc:94
+}
+
+int Engine::check(hipError_t e, const char *what, std::string *err)
+{
+    if (e == hipSuccess) return SPH_OK;
+    if (err) *err = std::string(what) + ": " + hipGetErrorString(e);
+    return SPH_ERR_HIP;
+}
+
+#define CK(expr, what)                                   \
+    do {                                                 \
+        int rc_ = check((expr), (what), err);            \
+        if (rc_) return rc_;                             \
+    } while (0)
+
+int Engine::create(const std::vector<SceneSpec> &scenes, uint32_t capacity,
+                   uint32_t table_capacity, uint32_t scene_capacity, Engine **out,
+                   std::string *err)
+{
+    if (scenes.empty()) {
+        *err = "no scenes";
+        return SPH_ERR_INVALID;
+    }
+    uint64_t n_total = 0, t_total = 0;
+    for (const SceneSpec &s : scenes) {
+        if (s.n == 0 || s.table == 0) {
+            *err = "scene needs n > 0 and table_size > 0";
+            return SPH_ERR_INVALID;
+        }
+        if (const char *m = check_params(s.params)) {
+            *err = m;
+            return SPH_ERR_INVALID;
+        }
+        n_total += s.n;
+        t_total += s.table;
+    }
+    if (n_total > (1ull << 31) || t_total > (1ull << 31)) {
+        *err = "particle or bucket total exceeds 2^31";
+        return SPH_ERR_INVALID;
+    }
+    capacity = std::max<uint32_t>(capacity, (uint32_t)n_total);
+    table_capacity = std::max<uint32_t>(table_capacity, (uint32_t)t_total);
+
+    Engine *e = new Engine();
+    int rc = e->check(hipStreamCreateWithFlags(&e->stream_, hipStreamNonBlocking),
+                      "hipStreamCreate", err);
+    e->scene_cap_ = std::max<uint32_t>(scene_capacity, (uint32_t)scenes.size());
+    if (!rc) rc = e->alloc(capacity, table_capacity, err);
+    if (rc) {
+        delete e;
+        return rc;
+    }
+    uint32_t p_off = 0, k_off = 0;
+    std::vector<uint32_t> scene_of(capacity, 0);
+    for (uint32_t s = 0; s < scenes.size(); ++s) {
+        SceneDesc d{};
+        fill_scene_desc(scenes[s].params, scenes[s].n, scenes[s].table, &d);
+        d.p_off = p_off;
+        d.key_off = k_off;
+        d.dt = 0.0f;
+        for (uint32_t i = 0; i < scenes[s].n; ++i) scene_of[p_off + i] = s;
+        p_off += scenes[s].n;
+        k_off += scenes[s].table;
+        e->sd_h_.push_back(d);
+        e->params_.push_back(scenes[s].params);
+    }
+    e->ed_h_->n_total = p_off;
+    e->ed_h_->table_total = k_off;
+    e->ed_h_->n_scenes = (uint32_t)scenes.size();
+    e->key_bits_ = bits_for(table_capacity - 1);
+    e->ed_h_->key_bits = e->key_bits_;
+    e->d_.multi = scenes.size() > 1;
+    rc = e->check(hipMemcpy(e->d_scene_of_, scene_of.data(), sizeof(uint32_t) * capacity,
+                            hipMemcpyHostToDevice),
+                  "upload scene map", err);
+    if (!rc) rc = e->push_descs(err);
+    if (!rc)
+        rc = e->check(hipMemsetAsync(e->d_.pos, 0, sizeof(float4) * capacity, e->stream_),
+                      "clear pos", err);
+    if (!rc)
+        rc = e->check(hipMemsetAsync(e->d_.vel, 0, sizeof(float4) * capacity, e->stream_),
+                      "clear vel", err);
+    if (!rc)
+        rc = e->check(hipMemsetAsync(e->d_.dens, 0, sizeof(float4) * capacity, e->stream_),
+                      "clear dens", err);
+    if (!rc) rc = e->check(hipStreamSynchronize(e->stream_), "sync", err);
+    for (int i = 0; i < 7 && !rc; ++i) rc = e->check(hipEventCreate(&e->ev_[i]), "event", err);
+    if (rc) {
+        delete e;
+        return rc;
+    }
+    *out = e;
+    return SPH_OK;
+}
+
+int Engine::alloc(uint32_t cap, uint32_t tcap, std::string *err)
+{
+    auto dm = [&](void **p, size_t bytes, const char *what) {
+        device_bytes_ += bytes;
+        return check(hipMalloc(p, bytes), what, err);
+    };
+    const size_t f4 = sizeof(float4) * (size_t)cap, u4 = sizeof(uint32_t) * (size_t)cap;
+    // radix blocks: each block owns at least 16 tiles of 256 keys
+    uint32_t nblk = (cap + 4095) / 4096;
+    nblk = std::max<uint32_t>(1, std::min<uint32_t>(nblk, 1024));
+    d_.radix_blocks = nblk;
+    d_.capacity = cap;
+    d_.table_capacity = tcap;
+    int rc = 0;
+    if (!rc) rc = check(hipHostMalloc((void **)&ed_h_, sizeof(EngineDesc)), "pinned desc", err);
+    if (!rc) rc = dm((void **)&d_ed_, sizeof(EngineDesc), "engine desc");
+    if (!rc) rc = dm((void **)&d_sd_, sizeof(SceneDesc) * scene_cap_, "scene table");
+    if (!rc)
+        rc = check(hipHostMalloc((void **)&sd_pinned_, sizeof(SceneDesc) * scene_cap_),
+                   "pinned scene table", err);
+    if (!rc) rc = dm((void **)&d_scene_of_, u4, "scene map");
+    if (!rc) rc = dm((void **)&d_.pos, f4, "pos");
+    if (!rc) rc = dm((void **)&d_.vel, f4, "vel");
+    if (!rc) rc = dm((void **)&d_.xp, f4, "xp");
+    if (!rc) rc = dm((void **)&d_.xps, f4, "xps");
+    if (!rc) rc = dm((void **)&d_.vs, f4, "vs");
+    if (!rc) rc = dm((void **)&d_.vs2, f4, "vs2");
+    if (!rc) rc = dm((void **)&d_.dens, f4, "dens");
+    for (int b = 0; b < 2 && !rc; ++b) {
+        rc = dm((void **)&d_.keys[b], u4, "keys");
+        if (!rc) rc = dm((void **)&d_.vals[b], u4, "vals");
+    }
+    if (!rc) rc = dm((void **)&d_.hist, sizeof(uint32_t) * 256 * (size_t)nblk, "radix hist");
+    if (!rc) rc = dm((void **)&d_.start, sizeof(uint32_t) * (size_t)tcap, "start table");
+    if (!rc) rc = dm((void **)&d_.end, sizeof(uint32_t) * (size_t)tcap, "end table");
+    d_.ed = d_ed_;
+    d_.sd = d_sd_;
+    d_.scene_of = d_scene_of_;
+    if (rc && rc == SPH_ERR_HIP && err) *err = "device allocation failed: " + *err;
+    return rc == SPH_ERR_HIP ? SPH_ERR_OOM : rc;
+}
+
+Engine::~Engine()
+{
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    invalidate_graph();
+    void *bufs[] = {d_ed_,       d_sd_,       d_scene_of_, d_.pos,     d_.vel,    d_.xp,
+                    d_.xps,      d_.vs,       d_.vs2,      d_.dens,    d_.keys[0], d_.keys[1],
+                    d_.vals[0],  d_.vals[1],  d_.hist,     d_.start,   d_.end};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    if (ed_h_) (void)hipHostFree(ed_h_);
+    if (sd_pinned_) (void)hipHostFree(sd_pinned_);
+    for (hipEvent_t &e : ev_)
+        if (e) (void)hipEventDestroy(e);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Engine::invalidate_graph()
+{
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    graph_exec_ = nullptr;
+    graph_ = nullptr;
+}
+
+int Engine::push_descs(std::string *err)
+{
+    // The pinned mirrors may still feed an earlier copy: drain the stream first.
+    CK(hipStreamSynchronize(stream_), "sync before descriptor update");
+    if (sd_h_.size() > scene_cap_) {
+        *err = "scene capacity exceeded";
+        return SPH_ERR_INVALID;
+    }
+    const size_t sd_bytes = sizeof(SceneDesc) * sd_h_.size();
+    memcpy(sd_pinned_, sd_h_.data(), sd_bytes);
+    if (sd_bytes)
+        CK(hipMemcpyAsync(d_sd_, sd_pinned_, sd_bytes, hipMemcpyHostToDevice, stream_),
+           "upload scene table");
+    CK(hipMemcpyAsync(d_ed_, ed_h_, sizeof(EngineDesc), hipMemcpyHostToDevice, stream_),
+       "upload engine desc");
+    return SPH_OK;
+}
+
+int Engine::set_dt(float dt, std::string *err)
+{
+    if (dt == dt_cached_) return SPH_OK;
+    for (SceneDesc &d : sd_h_) d.dt = dt;
+    dt_cached_ = dt;
+    return push_descs(err);
+}
+
+int Engine::set_params(uint32_t s, const sph_params &p, std::string *err)
+{
+    if (s >= sd_h_.size()) {
+        *err = "scene index out of range";
+        return SPH_ERR_INVALID;
+    }
+    if (const char *m = check_params(p)) {
+        *err = m;
+        return SPH_ERR_INVALID;
+    }
+    SceneDesc &d = sd_h_[s];
+    const float dt = d.dt;
+    const uint32_t p_off = d.p_off, k_off = d.key_off;
+    fill_scene_desc(p, d.n, d.table, &d);
+    d.p_off = p_off;
+    d.key_off = k_off;
+    d.dt = dt;
+    params_[s] = p;
+    return push_descs(err);
+}
+
+int Engine::upload(uint32_t s, const float *pos4, const float *vel4, std::string *err)
+{
+    if (s >= sd_h_.size()) {
+        *err = "scene index out of range";
+        return SPH_ERR_INVALID;
+    }
+    const SceneDesc &d = sd_h_[s];
+    const size_t bytes = sizeof(float4) * d.n;
+    if (pos4)
+        CK(hipMemcpyAsync(d_.pos + d.p_off, pos4, bytes, hipMemcpyHostToDevice, stream_),
+           "upload positions");
+    if (vel4)
+        CK(hipMemcpyAsync(d_.vel + d.p_off, vel4, bytes, hipMemcpyHostToDevice, stream_),
+           "upload velocities");
+    CK(hipStreamSynchronize(stream_), "sync after upload");
+    return SPH_OK;
+}
+
+hipError_t Engine::enqueue_update(hipEvent_t *ev)
+{
+    // ev (optional): 6 events bracketing predict | spatial | density |
+    // pressure | viscosity+integrate
+    hipError_t e;
+#define LK(x)                              \
+    if ((e = (x)) != hipSuccess) return e;
+    if (ev) LK(hipEventRecord(ev[0], stream_));
+    LK(launch_predict(d_, cfg_, stream_));
+    if (ev) LK(hipEventRecord(ev[1], stream_));
+    LK(launch_clear_table(d_, cfg_, stream_));
+    LK(launch_radix_sort(d_, key_bits_, stream_, &sorted_buf_));
+    LK(launch_reorder(d_, cfg_, sorted_buf_, stream_));
+    if (ev) LK(hipEventRecord(ev[2], stream_));
+    LK(launch_density(d_, cfg_, stream_));
+    if (ev) LK(hipEventRecord(ev[3], stream_));
+    LK(launch_pressure(d_, cfg_, stream_));
+    if (ev) LK(hipEventRecord(ev[4], stream_));
+    LK(launch_viscosity_integrate(d_, cfg_, sorted_buf_, stream_));
+    if (ev) LK(hipEventRecord(ev[5], stream_));
+#undef LK
+    return hipSuccess;
+}
+
+int Engine::update(float dt, std::string *err)
+{
+    if (!isfinite(dt)) {
+        *err = "dt must be finite";
+        return SPH_ERR_INVALID;
+    }
+    int rc = set_dt(dt, err);
+    if (rc) return rc;
+    if (prof_active_ && prof_used_ * 6 < prof_ev_.size()) {
+        CK(enqueue_update(&prof_ev_[6 * prof_used_]), "launch profiled update");
+        ++prof_used_;
+    } else if (timing_) {
+        CK(enqueue_update(ev_), "launch update");
+        CK(hipEventSynchronize(ev_[5]), "sync update");
+        for (int i = 0; i < 5; ++i) {
+            float ms = 0.0f;
+            CK(hipEventElapsedTime(&ms, ev_[i], ev_[i + 1]), "stage timer");
+            stage_ms_[i] = ms;
+        }
+        stage_ms_[5] = 0.0;  // position + collision runs inside the viscosity kernel
+    } else if (use_graph_) {
+        if (!graph_exec_) {
+            CK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture");
+            hipError_t le = enqueue_update(nullptr);
+            hipError_t ce = hipStreamEndCapture(stream_, &graph_);
+            CK(le, "capture update");
+            CK(ce, "end capture");
+            CK(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "instantiate graph");
+        }
+        CK(hipGraphLaunch(graph_exec_, stream_), "graph launch");
+    } else {
+        CK(enqueue_update(nullptr), "launch update");
+    }
+    have_tables_ = true;
+    return SPH_OK;
+}
+
+int Engine::profile_begin(uint32_t max_updates, std::string *err)
+{
+    if (prof_active_) {
+        *err = "profiling already active";
+        return SPH_ERR_STATE;
+    }
+    if (max_updates == 0 || max_updates > (1u << 20)) {
+        *err = "max_updates must be in [1, 2^20]";
+        return SPH_ERR_INVALID;
+    }
+    prof_ev_.assign(6 * (size_t)max_updates, nullptr);
+    for (hipEvent_t &e : prof_ev_) CK(hipEventCreate(&e), "profile event");
+    prof_used_ = 0;
+    prof_active_ = true;
+    return SPH_OK;
+}
+
+int Engine::profile_end(double mean_ms[6], uint32_t *updates, std::string *err)
+{
+    if (!prof_active_) {
+        *err = "profiling not active";
+        return SPH_ERR_STATE;
+    }
+    int rc = SPH_OK;
+    double sum[6] = {0, 0, 0, 0, 0, 0};
+    if (hipStreamSynchronize(stream_) != hipSuccess) rc = SPH_ERR_HIP;
+    for (uint32_t u = 0; u < prof_used_ && !rc; ++u)
+        for (int i = 0; i < 5; ++i) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, prof_ev_[6 * u + i], prof_ev_[6 * u + i + 1]) != hipSuccess) {
+                rc = SPH_ERR_HIP;
+                break;
+            }
+            sum[i] += ms;
+        }
+    for (int i = 0; i < 6; ++i) mean_ms[i] = prof_used_ ? sum[i] / prof_used_ : 0.0;
+    if (updates) *updates = prof_used_;
+    for (hipEvent_t &e : prof_ev_)
+        if (e) (void)hipEventDestroy(e);
+    prof_ev_.clear();
+    prof_active_ = false;
+    prof_used_ = 0;
+    if (rc) *err = "profile readout failed";
+    return rc;
+}
+
+int Engine::step(float dt, int substeps, bool sync_after, std::string *err)
+{
+    if (substeps < 1) {
+        *err = "substeps must be >= 1";
+        return SPH_ERR_INVALID;
+    }
+    const float sub = dt / (float)substeps;
+    for (int i = 0; i < substeps; ++i) {
+        int rc = update(sub, err);
+        if (rc) return rc;
+    }
+    return sync_after ? sync(err) : SPH_OK;
+}
+
+int Engine::sync(std::string *err)
+{
+    CK(hipStreamSynchronize(stream_), "sync");
+    return SPH_OK;
+}
+
+int Engine::prime(std::string *err)
+{
+    // InitializeData (cc:144-145) runs UpdateSpatialLookup + updateDensities on
+    // predictedPositions = positions (cc:550) with zero velocity: stage A with
+    // dt = 0 reproduces xp = x + 0 exactly.
+    int rc = set_dt(0.0f, err);
+    if (rc) return rc;
+    CK(launch_predict(d_, cfg_, stream_), "prime predict");
+    CK(launch_clear_table(d_, cfg_, stream_), "prime clear");
+    CK(launch_radix_sort(d_, key_bits_, stream_, &sorted_buf_), "prime sort");
+    CK(launch_reorder(d_, cfg_, sorted_buf_, stream_), "prime reorder");
+    CK(launch_density(d_, cfg_, stream_), "prime density");
+    CK(hipStreamSynchronize(stream_), "prime sync");
+    have_tables_ = true;
+    return SPH_OK;
+}
+
+int Engine::download(uint32_t s, float *pos4, float *vel4, float *dens2, std::string *err)
+{
+    if (s >= sd_h_.size()) {
+        *err = "scene index out of range";
+        return SPH_ERR_INVALID;
+    }
+    const SceneDesc &d = sd_h_[s];
+    CK(hipStreamSynchronize(stream_), "sync before download");
+    if (pos4)
+        CK(hipMemcpy(pos4, d_.pos + d.p_off, sizeof(float4) * d.n, hipMemcpyDeviceToHost),
+           "download positions");
+    if (vel4)
+        CK(hipMemcpy(vel4, d_.vel + d.p_off, sizeof(float4) * d.n, hipMemcpyDeviceToHost),
+           "download velocities");
+    if (dens2) {
+        if (!have_tables_) {
+            memset(dens2, 0, sizeof(float) * 2 * d.n);
+            return SPH_OK;
+        }
+        float2 *tmp = nullptr;
+        CK(hipMalloc((void **)&tmp, sizeof(float2) * d_.capacity), "density staging");
+        hipError_t e1 = launch_unpermute_dens(d_, sorted_buf_, tmp, stream_);
+        hipError_t e2 = e1 == hipSuccess ? hipStreamSynchronize(stream_) : e1;
+        hipError_t e3 = e2 == hipSuccess ? hipMemcpy(dens2, tmp + d.p_off, sizeof(float2) * d.n,
+                                                     hipMemcpyDeviceToHost)
+                                         : e2;
+        (void)hipFree(tmp);
+        CK(e3, "download densities");
+    }
+    return SPH_OK;
+}
+
+int Engine::debug_spatial(uint32_t s, uint32_t *hash_id, uint32_t *key_id, uint32_t *sorted_key,
+                          uint32_t *sorted_idx, uint32_t *start, uint32_t *end,
+                          std::string *err)
+{
+    if (s >= sd_h_.size()) {
+        *err = "scene index out of range";
+        return SPH_ERR_INVALID;
+    }
+    if (!have_tables_) {
+        *err = "no update has run yet";
+        return SPH_ERR_STATE;
+    }
+    const SceneDesc &d = sd_h_[s];
+    CK(hipStreamSynchronize(stream_), "sync");
+    std::vector<float4> xp(d.n);
+    std::vector<uint32_t> sk(d.n), si(d.n);
+    CK(hipMemcpy(xp.data(), d_.xp + d.p_off, sizeof(float4) * d.n, hipMemcpyDeviceToHost), "xp");
+    CK(hipMemcpy(sk.data(), d_.keys[sorted_buf_] + d.p_off, sizeof(uint32_t) * d.n,
+                 hipMemcpyDeviceToHost), "sorted keys");
+    CK(hipMemcpy(si.data(), d_.vals[sorted_buf_] + d.p_off, sizeof(uint32_t) * d.n,
+                 hipMemcpyDeviceToHost), "sorted idx");
+    for (uint32_t k = 0; k < d.n; ++k) {
+        sk[k] -= d.key_off;
+        si[k] -= d.p_off;
+    }
+    if (hash_id)
+        for (uint32_t i = 0; i < d.n; ++i) memcpy(&hash_id[i], &xp[i].w, 4);
+    if (key_id)
+        for (uint32_t k = 0; k < d.n; ++k) key_id[si[k] < d.n ? si[k] : 0] = sk[k];
+    if (sorted_key) memcpy(sorted_key, sk.data(), sizeof(uint32_t) * d.n);
+    if (sorted_idx) memcpy(sorted_idx, si.data(), sizeof(uint32_t) * d.n);
+    if (start || end) {
+        std::vector<uint32_t> st(d.table), en(d.table);
+        CK(hipMemcpy(st.data(), d_.start + d.key_off, sizeof(uint32_t) * d.table,
+                     hipMemcpyDeviceToHost), "start table");
+        CK(hipMemcpy(en.data(), d_.end + d.key_off, sizeof(uint32_t) * d.table,
+                     hipMemcpyDeviceToHost), "end table");
+        for (uint32_t b = 0; b < d.table; ++b) {
+            const bool empty = st[b] == SPH_START_EMPTY;
+            if (start) start[b] = empty ? SPH_START_EMPTY : st[b] - d.p_off;
+            if (end) end[b] = empty ? 0u : en[b] - d.p_off;
+        }
+    }
+    return SPH_OK;
+}
+
+int Engine::debug_predicted(uint32_t s, float *xp4, std::string *err)
+{
+    if (s >= sd_h_.size()) {
+        *err = "scene index out of range";
+        return SPH_ERR_INVALID;
+    }
+    const SceneDesc &d = sd_h_[s];
+    CK(hipStreamSynchronize(stream_), "sync");
+    CK(hipMemcpy(xp4, d_.xp + d.p_off, sizeof(float4) * d.n, hipMemcpyDeviceToHost), "xp");
+    for (uint32_t i = 0; i < d.n; ++i) xp4[4 * i + 3] = 0.0f;
+    return SPH_OK;
+}
+
+int Engine::debug_neighbours(uint32_t s, uint64_t *offs, uint32_t *list, uint64_t cap,
+                             uint64_t *total, std::string *err)
+{
+    if (s >= sd_h_.size()) {
+        *err = "scene index out of range";
+        return SPH_ERR_INVALID;
+    }
+    if (!have_tables_) {
+        *err = "no update has run yet";
+        return SPH_ERR_STATE;
+    }
+    const SceneDesc &d = sd_h_[s];
+    const uint32_t n_all = ed_h_->n_total;
+    uint32_t *d_counts = nullptr;
+    uint64_t *d_offs = nullptr;
+    uint32_t *d_list = nullptr;
+    std::vector<uint32_t> counts(n_all);
+    std::vector<uint64_t> offs_all(n_all + 1);
+    int rc = SPH_OK;
+    do {
+        if ((rc = check(hipMalloc((void **)&d_counts, sizeof(uint32_t) * n_all), "counts", err)))
+            break;
+        if ((rc = check(launch_neighbour_count(d_, sorted_buf_, d_counts, stream_), "count", err)))
+            break;
+        if ((rc = check(hipStreamSynchronize(stream_), "sync", err))) break;
+        if ((rc = check(hipMemcpy(counts.data(), d_counts, sizeof(uint32_t) * n_all,
+                                  hipMemcpyDeviceToHost), "counts", err)))
+            break;
+        offs_all[0] = 0;
+        for (uint32_t i = 0; i < n_all; ++i) offs_all[i + 1] = offs_all[i] + counts[i];
+        const uint64_t tot_all = offs_all[n_all];
+        if ((rc = check(hipMalloc((void **)&d_offs, sizeof(uint64_t) * (n_all + 1)), "offs", err)))
+            break;
+        if ((rc = check(hipMalloc((void **)&d_list, sizeof(uint32_t) * std::max<uint64_t>(tot_all, 1)),
+                        "list", err)))
+            break;
+        if ((rc = check(hipMemcpy(d_offs, offs_all.data(), sizeof(uint64_t) * (n_all + 1),
+                                  hipMemcpyHostToDevice), "offs", err)))
+            break;
+        if ((rc = check(launch_neighbour_fill(d_, sorted_buf_, d_offs, d_list, stream_), "fill", err)))
+            break;
+        if ((rc = check(hipStreamSynchronize(stream_), "sync", err))) break;
+        const uint64_t base = offs_all[d.p_off];
+        const uint64_t tot = offs_all[d.p_off + d.n] - base;
+        if (offs)
+            for (uint32_t i = 0; i <= d.n; ++i) offs[i] = offs_all[d.p_off + i] - base;
+        if (total) *total = tot;
+        if (list && cap) {
+            const uint64_t m = std::min(cap, tot);
+            if (m && (rc = check(hipMemcpy(list, d_list + base, sizeof(uint32_t) * m,
+                                           hipMemcpyDeviceToHost), "list", err)))
+                break;
+        }
+    } while (0);
+    if (d_counts) (void)hipFree(d_counts);
+    if (d_offs) (void)hipFree(d_offs);
+    if (d_list) (void)hipFree(d_list);
+    return rc;
+}
+
+int Engine::count_nonfinite(uint64_t *count, std::string *err)
+{
+    uint32_t *d_c = nullptr;
+    CK(hipMalloc((void **)&d_c, sizeof(uint32_t)), "counter");
+    uint32_t h = 0;
+    hipError_t e = hipMemsetAsync(d_c, 0, sizeof(uint32_t), stream_);
+    if (e == hipSuccess) e = launch_count_nonfinite(d_, d_c, stream_);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+    if (e == hipSuccess) e = hipMemcpy(&h, d_c, sizeof(uint32_t), hipMemcpyDeviceToHost);
+    (void)hipFree(d_c);
+    CK(e, "count non-finite");
+    *count = h;
+    return SPH_OK;
+}
+
+}  // namespace sph

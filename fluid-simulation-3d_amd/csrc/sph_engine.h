@@ -1,0 +1,112 @@
+// sph_engine.h -- host side of the gfx950 SPH step: owns the device buffers,
+// the HIP stream and the captured graph of one Update(dt) over S scenes.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/sph_amd.h"
+#include "sph_launch.h"
+#include "sph_types.h"
+
+namespace sph {
+
+struct SceneSpec {
+    sph_params params;
+    uint32_t n;
+    uint32_t table;
+};
+
+// Fills the derived constants of a SceneDesc from user settings, with the
+// reference's float expressions (kernels.h:25-82, physicsWorld.cc:47,87-88).
+void fill_scene_desc(const sph_params &p, uint32_t n, uint32_t table, SceneDesc *d);
+// Smallest float x >= 0 with sqrtf(x) >= h (so d < h <=> d*d-as-float < x).
+float sqrt_threshold(float h);
+// Validates settings; returns an error message or nullptr.
+const char *check_params(const sph_params &p);
+
+class Engine {
+  public:
+    // capacity: particle slots to allocate (>= total of scenes); table_capacity likewise.
+    static int create(const std::vector<SceneSpec> &scenes, uint32_t capacity,
+                      uint32_t table_capacity, uint32_t scene_capacity, Engine **out,
+                      std::string *err);
+    ~Engine();
+
+    uint32_t n_scenes() const { return (uint32_t)sd_h_.size(); }
+    uint32_t n_total() const { return ed_h_->n_total; }
+    const SceneDesc &scene(uint32_t s) const { return sd_h_[s]; }
+    const sph_params &params(uint32_t s) const { return params_[s]; }
+    hipStream_t stream() const { return stream_; }
+
+    int set_params(uint32_t s, const sph_params &p, std::string *err);
+    int upload(uint32_t s, const float *pos4, const float *vel4, std::string *err);
+    // One Update(dt) for every scene (physicsWorld.cc:39-111).
+    int update(float dt, std::string *err);
+    // `substeps` Update(dt / substeps) calls; blocks until done when sync.
+    int step(float dt, int substeps, bool sync, std::string *err);
+    int sync(std::string *err);
+    // InitializeData tail (cc:144-145): spatial lookup + densities of the current positions.
+    int prime(std::string *err);
+    int download(uint32_t s, float *pos4, float *vel4, float *dens2, std::string *err);
+    int debug_spatial(uint32_t s, uint32_t *hash_id, uint32_t *key_id, uint32_t *sorted_key,
+                      uint32_t *sorted_idx, uint32_t *start, uint32_t *end, std::string *err);
+    int debug_predicted(uint32_t s, float *xp4, std::string *err);
+    int debug_neighbours(uint32_t s, uint64_t *offs, uint32_t *list, uint64_t cap,
+                         uint64_t *total, std::string *err);
+    int count_nonfinite(uint64_t *count, std::string *err);
+
+    void set_timing(bool on) { timing_ = on; }
+    // Event pool for per-stage timing of up to `max_updates` updates without
+    // synchronising: stage boundaries are recorded on the engine's stream.
+    int profile_begin(uint32_t max_updates, std::string *err);
+    // Mean per-stage milliseconds over the profiled updates (stage order of
+    // stage_ms), their count, and the pool is released.
+    int profile_end(double mean_ms[6], uint32_t *updates, std::string *err);
+    const double *stage_ms() const { return stage_ms_; }
+    LaunchCfg &launch_cfg() { return cfg_; }
+    void invalidate_graph();
+    int set_use_graph(bool on) { use_graph_ = on; invalidate_graph(); return 0; }
+    // Mean duration (ms) of each gather kernel over the last timed update.
+    uint64_t device_bytes() const { return device_bytes_; }
+    const DevState &dev() const { return d_; }
+    int sorted_buf() const { return sorted_buf_; }
+
+  private:
+    Engine() = default;
+    int alloc(uint32_t capacity, uint32_t table_capacity, std::string *err);
+    int push_descs(std::string *err);
+    int set_dt(float dt, std::string *err);
+    hipError_t enqueue_update(hipEvent_t *ev);
+    int check(hipError_t e, const char *what, std::string *err);
+
+    hipStream_t stream_ = nullptr;
+    DevState d_{};
+    LaunchCfg cfg_{};
+    std::vector<SceneDesc> sd_h_;
+    std::vector<sph_params> params_;
+    EngineDesc *ed_h_ = nullptr;  // pinned mirror
+    SceneDesc *sd_pinned_ = nullptr;
+    EngineDesc *d_ed_ = nullptr;
+    SceneDesc *d_sd_ = nullptr;
+    uint32_t *d_scene_of_ = nullptr;
+    uint32_t key_bits_ = 1;
+    uint32_t scene_cap_ = 1;
+    int sorted_buf_ = 0;
+    bool have_tables_ = false;
+    bool use_graph_ = true;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t graph_exec_ = nullptr;
+    bool timing_ = false;
+    hipEvent_t ev_[7] = {};
+    double stage_ms_[6] = {0, 0, 0, 0, 0, 0};
+    uint64_t device_bytes_ = 0;
+    float dt_cached_ = -1.0f;
+    std::vector<hipEvent_t> prof_ev_;
+    uint32_t prof_used_ = 0;
+    bool prof_active_ = false;
+};
+
+}  // namespace sph

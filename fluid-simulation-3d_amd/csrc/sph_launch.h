@@ -1,0 +1,67 @@
+// sph_launch.h -- host-callable launchers of the gfx950 kernels in
+// sph_kernels.hip.  Every launcher enqueues on `stream` and returns the
+// launch status; none of them synchronises or allocates.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sph_types.h"
+
+namespace sph {
+
+struct DevState {
+    // engine descriptors
+    const EngineDesc *ed;
+    const SceneDesc *sd;
+    const uint32_t *scene_of;  // [n] scene of each slot (unused when single)
+    bool multi;                // more than one scene
+    // particle state, id order
+    float4 *pos;  // x,y,z, w = 0.34 after an update (OutPositions layout, cc:107)
+    float4 *vel;  // x,y,z, w unused
+    float4 *xp;   // predicted position, w = bits of the exact u32 hash
+    // sort
+    uint32_t *keys[2];
+    uint32_t *vals[2];
+    uint32_t *hist;        // [radix * radix_blocks]
+    uint32_t radix_blocks;
+    // sorted order
+    float4 *xps;   // predicted position, w = float(hash) (the reference record's .y)
+    float4 *vs;    // velocity after stage A, sorted
+    float4 *vs2;   // velocity after the pressure stage, sorted
+    float4 *dens;  // rho, rho_near, pressure, near pressure
+    uint32_t *start;  // [table_total] first sorted slot of each bucket
+    uint32_t *end;    // [table_total] one past the last slot (valid where start is)
+    uint32_t capacity;       // particle slots allocated
+    uint32_t table_capacity; // bucket slots allocated
+};
+
+struct LaunchCfg {
+    int block_gather = 256;  // threads per block of density / pressure / viscosity
+    int block_stream = 256;  // threads per block of streaming kernels
+};
+
+// Stage A + B1: predict, cell, hash, key; vals = slot index.
+hipError_t launch_predict(const DevState &s, const LaunchCfg &c, hipStream_t st);
+// Bucket table clear to SPH_START_EMPTY.
+hipError_t launch_clear_table(const DevState &s, const LaunchCfg &c, hipStream_t st);
+// Stage B2: stable LSD radix sort of (keys, vals) over `bits` key bits.
+// Returns the index (0/1) of the buffer pair holding the result.
+hipError_t launch_radix_sort(const DevState &s, uint32_t bits, hipStream_t st, int *result_buf);
+// Stage B3 + gather into sorted order.
+hipError_t launch_reorder(const DevState &s, const LaunchCfg &c, int buf, hipStream_t st);
+// Stage C, D, E+F.
+hipError_t launch_density(const DevState &s, const LaunchCfg &c, hipStream_t st);
+hipError_t launch_pressure(const DevState &s, const LaunchCfg &c, hipStream_t st);
+hipError_t launch_viscosity_integrate(const DevState &s, const LaunchCfg &c, int buf,
+                                      hipStream_t st);
+
+// Sorted -> id order densities (rho, rho_near) for download.
+hipError_t launch_unpermute_dens(const DevState &s, int buf, float2 *out, hipStream_t st);
+// Neighbour lists for parity checks (id order, walk order, self included).
+hipError_t launch_neighbour_count(const DevState &s, int buf, uint32_t *counts, hipStream_t st);
+hipError_t launch_neighbour_fill(const DevState &s, int buf, const uint64_t *offs,
+                                 uint32_t *list, hipStream_t st);
+// Count of non-finite position/velocity components (error flag).
+hipError_t launch_count_nonfinite(const DevState &s, uint32_t *counter, hipStream_t st);
+
+}  // namespace sph

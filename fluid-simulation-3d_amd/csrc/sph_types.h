@@ -1,0 +1,46 @@
+// sph_types.h -- data structures shared by host code and the gfx950 kernels.
+//
+// One "engine" holds S >= 1 independent scenes packed into flat SoA arrays.
+// Scene s owns particle slots [p_off, p_off + n) -- the same range in id order
+// and in sorted order, because every key of scene s lies in
+// [key_off, key_off + table) and the key ranges of scenes are disjoint and
+// increasing with s.  A single-scene handle is an engine with S = 1.
+#pragma once
+#include <stdint.h>
+
+#define SPH_START_EMPTY 0x7fffffffu  // INT_MAX: empty bucket (physicsWorld.cc:481)
+
+// Per-scene constants, resident in device memory and read by every kernel.
+// Everything derived from the user settings (kernel normalisations, the
+// remainder constant, the d < h threshold on d^2) is computed once on the host
+// with the same float expressions as the reference (kernels.h:25-82).
+struct SceneDesc {
+    uint32_t p_off;       // first particle slot
+    uint32_t n;           // particle count
+    uint32_t key_off;     // first bucket in the global start/end tables
+    uint32_t table;       // bucket count T (reference: numParticles, cc:479)
+    uint64_t mod_m;       // Lemire remainder constant: floor((2^64-1)/table)+1
+    float h;              // interactionRadius
+    float r2;             // distance prefilter sqrRadius (physicsWorld.h:96)
+    float h2_lt;          // smallest float x with sqrtf(x) >= h: d < h  <=>  d2 < h2_lt
+    float rho0, k, k_near, mu;
+    float w2c, w3c;       // SmoothingPow2/Pow3 volumes (kernels.h:29, 41)
+    float dw2c, dw3c;     // derivative scales (kernels.h:53, 65)
+    float poly6c;         // viscosity poly6 scale (kernels.h:77)
+    float hh;             // h*h (kernels.h:78: radius * radius)
+    float g[3];           // external acceleration (cc:313-323)
+    float lookahead;      // 1/120 (cc:47)
+    float half[3];        // BoundScale * 0.5f (cc:88)
+    float damp_neg;       // -1 * dampFactor (cc:94)
+    float dt;             // substep dt of the current Update
+    uint32_t pad;
+};
+
+// Whole-engine counters, resident in device memory so that a captured graph
+// stays valid while the packed layout changes (continuous batching).
+struct EngineDesc {
+    uint32_t n_total;      // particles over all scenes
+    uint32_t table_total;  // buckets over all scenes
+    uint32_t n_scenes;
+    uint32_t key_bits;     // significant bits of the largest key (radix passes)
+};

@@ -1,0 +1,283 @@
+"""Python binding of libsph_amd.so (the C ABI in include/sph_amd.h).
+
+The SPH step itself runs in the hand-written gfx950 kernels of libsph_amd.so;
+this module only marshals numpy arrays across the C ABI.  There is no CPU
+fallback: every call that needs the GPU fails with SphError when the library or
+the device is missing.
+
+Mirrors the reference's FluidSimulation interface (Allkams/Fluid-Simulation-3D,
+engine/physics/physicsWorld.h:28-155): `Sim.update(dt)` is Update(dt)
+(physicsWorld.cc:39-111), `Sim.initialize_data()` is InitializeData
+(cc:112-147), the settings fields are the get/set pairs of physicsWorld.h:52-77.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG_DIR, "libsph_amd.so")
+HEADER_PATH = os.path.join(os.path.dirname(PKG_DIR), "include", "sph_amd.h")
+
+SPH_OK = 0
+SPH_ERR_INVALID = -1
+SPH_ERR_HIP = -2
+SPH_ERR_OOM = -3
+SPH_ERR_BUDGET = -4
+SPH_ERR_STATE = -5
+SPH_ERR_NOT_FOUND = -6
+SPH_ERR_NONFINITE = -7
+START_EMPTY = 0x7FFFFFFF  # INT_MAX, physicsWorld.cc:481
+
+INIT_GRID_ARRANGEMENT = 0
+INIT_DAM_BREAK = 1
+INIT_UNIFORM_LOWER = 2
+INIT_TWO_BLOCKS = 3
+INIT_CENTRED_BLOCK = 4
+
+
+class SphError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"sph error {code}: {msg}")
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    """sph_params: the reference's settings (physicsWorld.h:96-106,145)."""
+
+    _fields_ = [
+        ("smoothing_radius", ctypes.c_float),
+        ("target_density", ctypes.c_float),
+        ("pressure_mult", ctypes.c_float),
+        ("near_pressure_mult", ctypes.c_float),
+        ("viscosity", ctypes.c_float),
+        ("gravity_on", ctypes.c_int32),
+        ("gravity_scale", ctypes.c_float),
+        ("bounds", ctypes.c_float * 3),
+        ("lookahead", ctypes.c_float),
+        ("damp", ctypes.c_float),
+        ("prefilter_r2", ctypes.c_float),
+    ]
+
+    def as_dict(self):
+        d = {f: getattr(self, f) for f, _ in self._fields_}
+        d["bounds"] = tuple(self.bounds)
+        return d
+
+
+def build():
+    """Compile libsph_amd.so (hipcc, gfx950) with the package Makefile."""
+    subprocess.run(["make", "-s", "-j8", "-C", PKG_DIR], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise SphError(SPH_ERR_STATE, f"{LIB_PATH} is not built (run __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    i32, u32, u64, f, p = ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_float, ctypes.c_void_p
+    H = ctypes.c_void_p
+    pp = ctypes.POINTER(Params)
+    sig = {
+        "sph_default_params": (None, [pp]),
+        "sph_last_error": (ctypes.c_char_p, []),
+        "sph_version": (ctypes.c_char_p, []),
+        "sph_device_count": (i32, []),
+        "sph_create": (i32, [pp, u32, u32, ctypes.POINTER(H)]),
+        "sph_destroy": (i32, [H]),
+        "sph_set_params": (i32, [H, pp]),
+        "sph_get_params": (i32, [H, pp]),
+        "sph_get_size": (i32, [H, p, p]),
+        "sph_upload": (i32, [H, p, p]),
+        "sph_initialize_data": (i32, [H]),
+        "sph_update": (i32, [H, f]),
+        "sph_step": (i32, [H, f, i32]),
+        "sph_step_async": (i32, [H, f, i32]),
+        "sph_sync": (i32, [H]),
+        "sph_download": (i32, [H, p, p, p]),
+        "sph_set_timing": (i32, [H, i32]),
+        "sph_stage_ms": (i32, [H, p]),
+        "sph_count_nonfinite": (i32, [H, p]),
+        "sph_profile_begin": (i32, [H, u32]),
+        "sph_profile_end": (i32, [H, p, p]),
+        "sph_debug_spatial": (i32, [H, p, p, p, p, p, p]),
+        "sph_debug_predicted": (i32, [H, p]),
+        "sph_debug_neighbours": (i32, [H, p, p, u64, p]),
+        "sph_generate": (i32, [i32, u64, u32, p, f, f, f, p, p]),
+        "sph_hash_cell": (u32, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+        "sph_bucket_of": (u32, [u32, u32]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def last_error():
+    return lib().sph_last_error().decode()
+
+
+def _check(rc):
+    if rc != SPH_OK:
+        raise SphError(rc, last_error())
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def default_params(**kw):
+    p = Params()
+    lib().sph_default_params(ctypes.byref(p))
+    for k, v in kw.items():
+        if k == "bounds":
+            p.bounds[:] = [float(x) for x in v]
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def device_count():
+    return lib().sph_device_count()
+
+
+def generate(kind, seed, n, box=(1.0, 1.0, 1.0), spacing=0.1, jitter=0.0, speed=0.0):
+    """Deterministic synthetic scene: (pos4, vel4) float32 arrays of shape (n, 4)."""
+    pos = np.zeros((n, 4), np.float32)
+    vel = np.zeros((n, 4), np.float32)
+    b = np.asarray(box, np.float32)
+    _check(lib().sph_generate(kind, seed, n, _ptr(b), spacing, jitter, speed, _ptr(pos), _ptr(vel)))
+    return pos, vel
+
+
+def hash_cell(cx, cy, cz):
+    """HashCell (physicsWorld.cc:505-511) as the kernels compute it."""
+    return lib().sph_hash_cell(int(cx), int(cy), int(cz))
+
+
+def bucket_of(hash_, table):
+    """GetKeyFromHash (physicsWorld.cc:513-516) as the kernels compute it."""
+    return lib().sph_bucket_of(hash_, table)
+
+
+class Sim:
+    """One scene on the GPU: the FluidSimulation of physicsWorld.h:28-155."""
+
+    def __init__(self, params, n, table_size=0):
+        self._h = ctypes.c_void_p()
+        self.n = int(n)
+        _check(lib().sph_create(ctypes.byref(params), self.n, table_size, ctypes.byref(self._h)))
+        t = np.zeros(1, np.uint32)
+        lib().sph_get_size(self._h, None, _ptr(t))
+        self.table = int(t[0])
+
+    def close(self):
+        if self._h:
+            lib().sph_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def params(self):
+        p = Params()
+        _check(lib().sph_get_params(self._h, ctypes.byref(p)))
+        return p
+
+    @params.setter
+    def params(self, p):
+        _check(lib().sph_set_params(self._h, ctypes.byref(p)))
+
+    def upload(self, pos4=None, vel4=None):
+        pos = None if pos4 is None else np.ascontiguousarray(pos4, np.float32)
+        vel = None if vel4 is None else np.ascontiguousarray(vel4, np.float32)
+        for a in (pos, vel):
+            if a is not None and a.shape != (self.n, 4):
+                raise ValueError(f"expected shape {(self.n, 4)}, got {a.shape}")
+        _check(lib().sph_upload(self._h, _ptr(pos), _ptr(vel)))
+
+    def initialize_data(self):
+        _check(lib().sph_initialize_data(self._h))
+
+    def update(self, dt):
+        _check(lib().sph_update(self._h, dt))
+
+    def step(self, dt, substeps=1):
+        _check(lib().sph_step(self._h, dt, substeps))
+
+    def step_async(self, dt, substeps=1):
+        _check(lib().sph_step_async(self._h, dt, substeps))
+
+    def sync(self):
+        _check(lib().sph_sync(self._h))
+
+    def download(self):
+        pos = np.zeros((self.n, 4), np.float32)
+        vel = np.zeros((self.n, 4), np.float32)
+        dens = np.zeros((self.n, 2), np.float32)
+        _check(lib().sph_download(self._h, _ptr(pos), _ptr(vel), _ptr(dens)))
+        return pos, vel, dens
+
+    def set_timing(self, on):
+        _check(lib().sph_set_timing(self._h, 1 if on else 0))
+
+    def stage_ms(self):
+        ms = np.zeros(6, np.float64)
+        _check(lib().sph_stage_ms(self._h, _ptr(ms)))
+        return ms
+
+    def profile_begin(self, max_updates):
+        _check(lib().sph_profile_begin(self._h, int(max_updates)))
+
+    def profile_end(self):
+        ms = np.zeros(6, np.float64)
+        u = np.zeros(1, np.uint32)
+        _check(lib().sph_profile_end(self._h, _ptr(ms), _ptr(u)))
+        return ms, int(u[0])
+
+    def count_nonfinite(self):
+        c = np.zeros(1, np.uint64)
+        _check(lib().sph_count_nonfinite(self._h, _ptr(c)))
+        return int(c[0])
+
+    def debug_spatial(self):
+        n, T = self.n, self.table
+        out = {k: np.zeros(n, np.uint32) for k in ("hash_id", "key_id", "sorted_key", "sorted_idx")}
+        out["start"] = np.zeros(T, np.uint32)
+        out["end"] = np.zeros(T, np.uint32)
+        _check(lib().sph_debug_spatial(self._h, _ptr(out["hash_id"]), _ptr(out["key_id"]),
+                                       _ptr(out["sorted_key"]), _ptr(out["sorted_idx"]),
+                                       _ptr(out["start"]), _ptr(out["end"])))
+        return out
+
+    def debug_predicted(self):
+        xp = np.zeros((self.n, 4), np.float32)
+        _check(lib().sph_debug_predicted(self._h, _ptr(xp)))
+        return xp
+
+    def debug_neighbours(self):
+        offs = np.zeros(self.n + 1, np.uint64)
+        total = np.zeros(1, np.uint64)
+        _check(lib().sph_debug_neighbours(self._h, _ptr(offs), None, 0, _ptr(total)))
+        lst = np.zeros(max(int(total[0]), 1), np.uint32)
+        _check(lib().sph_debug_neighbours(self._h, _ptr(offs), _ptr(lst), lst.size, _ptr(total)))
+        return offs, lst[: int(total[0])]

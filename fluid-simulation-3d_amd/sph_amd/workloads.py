@@ -1,0 +1,69 @@
+"""The benchmark scenes, as (params, n, table_size, pos4, vel4, dt, substeps, steps).
+
+Scene definitions (all FP32, positions from sph_generate's counter-based
+generator, so every consumer sees identical bytes):
+
+  dam_break_16k   16,384 particles, dam-break block 16x32x32 on a lattice of
+                  spacing 0.1 (= 0.5 h) at the -x,-y,-z corner of a 4x4x4 box,
+                  jitter +-0.002 (= 0.01 h), v = 0, h = 0.2, gravity 9.81,
+                  dt = 1/120 in 3 substeps, viscosity 0.05, 200 steps.
+  uniform_1m      1,048,576 particles uniform in the lower half of a 16x8x16
+                  box, table 2^21, same settings, 1,000 steps.
+  two_blocks_16m  16,777,216 particles in two lattice blocks moving at +-2 m/s
+                  along x in a 40x20x40 box, h = 0.2, 500 steps.
+
+Settings shared by all three: pressure multiplier 300 and near-pressure
+multiplier 20 (the reference's defaults, physicsWorld.h:99-100), target
+density 1000 (the number density of a lattice of spacing 0.1, against which
+the normalised kernels sum to about 1000), the reference's fixed lookahead
+1/120, damping 0.95 and prefilter 0.35^2 (physicsWorld.cc:47, 87; h:96).
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import (INIT_DAM_BREAK, INIT_TWO_BLOCKS, INIT_UNIFORM_LOWER, default_params,
+               generate)
+
+DT = 1.0 / 120.0
+SUBSTEPS = 3
+
+
+def scene_params(h=0.2, rho0=1000.0, mu=0.05, box=(4.0, 4.0, 4.0), gravity=True):
+    return default_params(smoothing_radius=h, target_density=rho0, pressure_mult=300.0,
+                          near_pressure_mult=20.0, viscosity=mu, gravity_on=1 if gravity else 0,
+                          gravity_scale=9.81, bounds=box)
+
+
+@dataclass
+class Scene:
+    name: str
+    params: object
+    n: int
+    table: int
+    pos: np.ndarray
+    vel: np.ndarray
+    steps: int
+    dt: float = DT
+    substeps: int = SUBSTEPS
+    meta: dict = field(default_factory=dict)
+
+
+def dam_break(n=16384, seed=0, h=0.2, steps=200, box=(4.0, 4.0, 4.0)):
+    pos, vel = generate(INIT_DAM_BREAK, seed, n, box, 0.5 * h, 0.01 * h)
+    return Scene("dam_break_%d" % n, scene_params(h=h, box=box), n, n, pos, vel, steps)
+
+
+def uniform_lower(n=1 << 20, table=1 << 21, seed=1, h=0.2, steps=1000, box=(16.0, 8.0, 16.0)):
+    pos, vel = generate(INIT_UNIFORM_LOWER, seed, n, box)
+    return Scene("uniform_%d" % n, scene_params(h=h, box=box), n, table, pos, vel, steps)
+
+
+def two_blocks(n=1 << 24, seed=2, h=0.2, steps=500, box=(40.0, 20.0, 40.0), speed=2.0):
+    pos, vel = generate(INIT_TWO_BLOCKS, seed, n, box, 0.5 * h, 0.01 * h, speed)
+    return Scene("two_blocks_%d" % n, scene_params(h=h, box=box), n, n, pos, vel, steps)
+
+
+def by_name(name):
+    return {"dam_break_16k": dam_break, "uniform_1m": uniform_lower,
+            "two_blocks_16m": two_blocks}[name]()

@@ -1,0 +1,172 @@
+/*
+ * sph_amd.h -- C ABI of the MI355X (gfx950) SPH step.
+ *
+ * Drop-in for the SPH step of Allkams/Fluid-Simulation-3D: the singleton
+ * Physics::Fluid::FluidSimulation (engine/physics/physicsWorld.h:28-155,
+ * physicsWorld.cc:32-558) and its FluidSimBase backend slot
+ * (projects/Simulation/code/simulations/fluidSimBase.h:5-13).
+ *
+ * Conventions
+ *   - Every function returns SPH_OK (0) or a negative SPH_ERR_* code; the
+ *     message of the last failure on the calling thread is sph_last_error().
+ *     No exception crosses the ABI.
+ *   - Handles are independent (no singleton); each owns its device buffers and
+ *     one HIP stream.  Calls on one handle must be serialised by the caller;
+ *     different handles may be used from different threads.
+ *   - Host buffers are caller-owned.  Particle arrays are float4 per particle
+ *     (x, y, z, w): pos4 / vel4 hold 4*n floats.  After an update pos4.w is
+ *     0.34, so pos4 is also the reference's OutPositions (physicsWorld.h:80,
+ *     physicsWorld.cc:107).
+ *   - Results follow the reference's FP32 operation order, with two stated
+ *     differences: the sort is stable (equal keys keep particle-index order,
+ *     where std::sort at physicsWorld.cc:484 leaves it unspecified) and the
+ *     viscosity stage reads a velocity snapshot (Jacobi) where the reference
+ *     reads velocity[j] in place (physicsWorld.cc:460).
+ */
+#ifndef SPH_AMD_H
+#define SPH_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPH_OK 0
+#define SPH_ERR_INVALID (-1)   /* bad argument or settings */
+#define SPH_ERR_HIP (-2)       /* HIP runtime failure (message has the HIP error) */
+#define SPH_ERR_OOM (-3)       /* device or host allocation failed */
+#define SPH_ERR_BUDGET (-4)    /* request can never fit the server's HBM budget */
+#define SPH_ERR_STATE (-5)     /* call not valid in the current state */
+#define SPH_ERR_NOT_FOUND (-6) /* unknown request id */
+#define SPH_ERR_NONFINITE (-7) /* state holds NaN/Inf */
+
+/* Simulation settings.  Field <-> reference member (physicsWorld.h):
+ *   smoothing_radius   interactionRadius       :97  (setInteractionRadius :58)
+ *   target_density     TargetDensity           :98  (setDensityTarget :61)
+ *   pressure_mult      pressureMultiplier      :99  (setPressureMultiplier :64)
+ *   near_pressure_mult nearPressureMultiplier  :100 (setNearPressureMultiplier :67)
+ *   viscosity          viscosityStrength       :101 (setViscosityStrength :70)
+ *   gravity_on         gravity                 :105 (setGravity :55)
+ *   gravity_scale      gravityScale            :106 (setGravityScale :73)
+ *   bounds             BoundScale              :145 (setBound :76)
+ *   lookahead          1.0f/120.0f             physicsWorld.cc:47 (fixed in the reference)
+ *   damp               dampFactor 0.95f        physicsWorld.cc:87 (fixed in the reference)
+ *   prefilter_r2       sqrRadius 0.35f*0.35f   :96  (const in the reference) */
+typedef struct sph_params {
+    float smoothing_radius;
+    float target_density;
+    float pressure_mult;
+    float near_pressure_mult;
+    float viscosity;
+    int32_t gravity_on;
+    float gravity_scale;
+    float bounds[3];
+    float lookahead;
+    float damp;
+    float prefilter_r2;
+} sph_params;
+
+typedef struct sph_sim *sph_handle;
+
+/* The reference's in-class defaults (physicsWorld.h:96-106, 145). */
+void sph_default_params(sph_params *p);
+/* Message of the last failure on this thread ("" if none). */
+const char *sph_last_error(void);
+const char *sph_version(void);
+/* Number of visible HIP devices (0 without a GPU; never fails). */
+int sph_device_count(void);
+
+/* ---- single scene: FluidSimulation (physicsWorld.h:28-155) ---- */
+
+/* FluidSimulation::getInstance + InitializeData's sizing (h:32,36; cc:112-137):
+ * a scene of n particles whose spatial hash has table_size buckets
+ * (0 = n, the reference's choice: GetKeyFromHash(hash, numParticles), cc:479).
+ * All state starts at zero. */
+int sph_create(const sph_params *p, uint32_t n, uint32_t table_size, sph_handle *out);
+int sph_destroy(sph_handle h);
+/* The setters / getters of physicsWorld.h:52-77 (take effect at the next update). */
+int sph_set_params(sph_handle h, const sph_params *p);
+int sph_get_params(sph_handle h, sph_params *p);
+int sph_get_size(sph_handle h, uint32_t *n, uint32_t *table_size);
+/* Writes the positions vector (h:79) and the velocities; either may be NULL
+ * (left unchanged).  Host float4 arrays of n entries. */
+int sph_upload(sph_handle h, const float *pos4, const float *vel4);
+/* InitializeData(n) (cc:112-147): the reference's GridArrangement lattice
+ * (cc:518-557), zero velocity, then the spatial lookup and densities of that
+ * state (cc:144-145). */
+int sph_initialize_data(sph_handle h);
+/* Update(deltatime) (cc:39-111), blocking. */
+int sph_update(sph_handle h, float dt);
+/* `substeps` calls of Update(dt / substeps); blocking. */
+int sph_step(sph_handle h, float dt, int substeps);
+/* As sph_step, but returns once the work is enqueued on the handle's stream. */
+int sph_step_async(sph_handle h, float dt, int substeps);
+int sph_sync(sph_handle h);
+/* Reads positions (= OutPositions, w = 0.34), velocities and the (density,
+ * near density) pairs of the last update, all in particle-index order; any
+ * pointer may be NULL.  Replaces getPosition/getVelocity/getDensity/
+ * getNearDensity (h:38-41, cc:149-170) and the OutPositions vector (h:80). */
+int sph_download(sph_handle h, float *pos4, float *vel4, float *dens2);
+/* The six stage timers of the last update in ms, physicsWorld.h:45-50 order:
+ * gravity, spatial, density, pressure, viscosity, position+collision.
+ * Filled only while timing is on (sph_set_timing); the viscosity and
+ * position+collision stages run as one kernel, reported under viscosity
+ * (position+collision reads 0). */
+int sph_set_timing(sph_handle h, int on);
+int sph_stage_ms(sph_handle h, double ms[6]);
+/* Per-stage device timing of the next `max_updates` updates (up to 2^20)
+ * without synchronising: events bracket each stage on the handle's stream.
+ * sph_profile_end waits, returns the mean ms per update of each stage (order
+ * of sph_stage_ms; the kernels are: 0 predict+hash, 1 table clear + radix sort
+ * + reorder, 2 density, 3 pressure, 4 viscosity+integrate) and the number of
+ * updates profiled.  No reference counterpart: measurement support for the
+ * stage timers of physicsWorld.h:45-50. */
+int sph_profile_begin(sph_handle h, uint32_t max_updates);
+int sph_profile_end(sph_handle h, double mean_ms[6], uint32_t *updates);
+/* Number of non-finite position/velocity components in the current state. */
+int sph_count_nonfinite(sph_handle h, uint64_t *count);
+
+/* ---- parity / debugging: the spatial lookup of the last update ----
+ * UpdateSpatialLookup (cc:466-498): per particle hash (HashCell) and key
+ * (GetKeyFromHash); per sorted slot key and particle index; per bucket the
+ * first sorted slot (startIndices, INT_MAX when empty) and one past the last.
+ * Any pointer may be NULL.  hash_id/key_id/sorted_* hold n entries,
+ * start/end table_size entries. */
+int sph_debug_spatial(sph_handle h, uint32_t *hash_id, uint32_t *key_id,
+                      uint32_t *sorted_key, uint32_t *sorted_idx, uint32_t *start,
+                      uint32_t *end);
+/* predictedPositions of the last update (physicsWorld.h:118), float4, w = 0. */
+int sph_debug_predicted(sph_handle h, float *xp4);
+/* Neighbour sets of the last update as the reference's bucket walk sees them
+ * (cc:331-363): for particle i, every j in the 27 cells whose hash matches and
+ * whose squared distance is <= prefilter_r2, self included, in walk order.
+ * CSR: offs[n+1]; list gets min(total, cap) entries; *total = offs[n]. */
+int sph_debug_neighbours(sph_handle h, uint64_t *offs, uint32_t *list, uint64_t cap,
+                         uint64_t *total);
+
+/* ---- the spatial hash as the kernels compute it (host copies, for callers
+ * and checks): HashCell (cc:505-511) on integer cell coordinates and
+ * GetKeyFromHash (cc:513-516), i.e. hash % table_size. ---- */
+uint32_t sph_hash_cell(int32_t cx, int32_t cy, int32_t cz);
+uint32_t sph_bucket_of(uint32_t hash, uint32_t table_size);
+
+/* ---- synthetic scenes (host only; identical bytes for every consumer) ---- */
+#define SPH_INIT_GRID_ARRANGEMENT 0 /* the reference's InitializeData lattice, cc:518-557 */
+#define SPH_INIT_DAM_BREAK 1        /* block at the -x,-y,-z corner, dims nx:2nx:2nx, z fastest */
+#define SPH_INIT_UNIFORM_LOWER 2    /* uniform in the lower half of the box */
+#define SPH_INIT_TWO_BLOCKS 3       /* two blocks moving at +-speed along x */
+#define SPH_INIT_CENTRED_BLOCK 4    /* cubic block centred in the box */
+/* Writes n float4 positions (w = 0.34) and velocities (w = 0).  box: full
+ * extent (the box is centred on the origin, as cc:88-106 assumes); spacing:
+ * lattice spacing; jitter: half-width of the uniform per-component jitter;
+ * speed: |v| of SPH_INIT_TWO_BLOCKS.  A counter-based generator (splitmix64 of
+ * seed and particle index) makes the bytes independent of threading. */
+int sph_generate(int kind, uint64_t seed, uint32_t n, const float box[3], float spacing,
+                 float jitter, float speed, float *pos4, float *vel4);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

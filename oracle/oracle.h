@@ -1,0 +1,161 @@
+/*
+ * oracle.h -- CPU restatement of the reference SPH step, used ONLY as a
+ * checker by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
+ *
+ * TEST INFRASTRUCTURE, NOT PRODUCT CODE.  Nothing in the product library
+ * (fluid-simulation-3d_amd/) links, loads or calls this file.
+ *
+ * What it restates (reference = Allkams/Fluid-Simulation-3D):
+ *   engine/physics/physicsWorld.cc:39-111   FluidSimulation::Update (stages A-F)
+ *   engine/physics/physicsWorld.cc:304-311  updateDensities
+ *   engine/physics/physicsWorld.cc:313-323  CalculateExternalFoce
+ *   engine/physics/physicsWorld.cc:325-365  CalculateDensity
+ *   engine/physics/physicsWorld.cc:367-422  CalculatePressureForce
+ *   engine/physics/physicsWorld.cc:424-464  CalculateViscosityForce
+ *   engine/physics/physicsWorld.cc:466-516  UpdateSpatialLookup, PositionToCellCoord,
+ *                                           HashCell, GetKeyFromHash
+ *   engine/physics/physicsWorld.cc:518-557  GridArrangement
+ *   engine/physics/physicsWorld.h:96-145    settings defaults, offsets[27]
+ *   engine/physics/kernels.h:25-82          the five smoothing kernels
+ *   exts/glm detail/type_vec3.inl:715-721   vec3/scalar = per-component division
+ *   exts/glm detail/func_geometric.inl:48-55 dot = (x*x + y*y) + z*z, products rounded
+ *
+ * FP32 functions keep the reference's operation order and must be compiled
+ * without FMA contraction (-ffp-contract=off) and without fast-math.
+ *
+ * Deliberate, documented choices of the restatement:
+ *   - the sort (physicsWorld.cc:484, an unstable std::sort on the float key)
+ *     is restated as a STABLE counting sort, i.e. (key, particle index)
+ *     order.  Keys and the per-bucket index SETS are what the reference
+ *     fixes; the within-bucket order of std::sort is a library artefact.
+ *   - viscosity (physicsWorld.cc:424-464) reads velocity[j] in place.  With
+ *     a serial PSTL backend that is a Gauss-Seidel sweep in index order;
+ *     orc_stage_viscosity(..., gauss_seidel=1) restates exactly that.  With
+ *     gauss_seidel=0 it is the Jacobi form (all reads from the snapshot
+ *     taken after the pressure stage), which is what the GPU computes.
+ *
+ * Parity status of each function is listed in oracle/README.md.
+ */
+#ifndef SPH_ORACLE_H
+#define SPH_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_START_EMPTY 0x7fffffffu /* INT_MAX, physicsWorld.cc:481 */
+
+typedef struct orc_params {
+    float h;            /* interactionRadius        physicsWorld.h:97  */
+    float rho0;         /* TargetDensity            physicsWorld.h:98  */
+    float k;            /* pressureMultiplier       physicsWorld.h:99  */
+    float k_near;       /* nearPressureMultiplier   physicsWorld.h:100 */
+    float mu;           /* viscosityStrength        physicsWorld.h:101 */
+    int32_t gravity_on; /* gravity                  physicsWorld.h:105 */
+    float gravity_scale;/* gravityScale             physicsWorld.h:106 */
+    float bounds[3];    /* BoundScale               physicsWorld.h:145 */
+    float lookahead;    /* 1.0f/120.0f              physicsWorld.cc:47 */
+    float damp;         /* dampFactor 0.95f         physicsWorld.cc:87 */
+    float r2;           /* sqrRadius 0.35f*0.35f    physicsWorld.h:96  */
+} orc_params;
+
+/* Reference defaults (physicsWorld.h:96-106,145; physicsWorld.cc:47,87). */
+void orc_default_params(orc_params *p);
+
+/* ---- kernels.h:25-82, evaluated per call exactly as the reference ---- */
+float orc_W2(float dist, float h);      /* SmoothingPow2            kernels.h:25-34 */
+float orc_W3(float dist, float h);      /* SmoothingPow3            kernels.h:37-46 */
+float orc_dW2(float dist, float h);     /* SmoothingDerivativePow2  kernels.h:49-58 */
+float orc_dW3(float dist, float h);     /* SmoothingDerivativePow3  kernels.h:61-70 */
+float orc_poly6(float dist, float h);   /* SmoothingViscoPoly6      kernels.h:73-82 */
+
+/* ---- spatial hash, physicsWorld.cc:499-516 ---- */
+void orc_cell_coord(const float xp[3], float h, float cell[3]);
+uint32_t orc_hash_cell(const float cell[3]);
+uint32_t orc_key(uint32_t hash, uint32_t table_size);
+
+/* ---- stages of Update (physicsWorld.cc:39-111) over float4 (x,y,z,w) arrays ---- */
+/* Stage A, cc:42-48: vel += g*dt; xp = pos + vel*lookahead.  xp4[.w] = 0. */
+void orc_stage_predict(uint32_t n, const orc_params *p, float dt,
+                       const float *pos4, float *vel4, float *xp4);
+
+/* Stage B, cc:466-498 with a stable counting sort.  Outputs (any may be NULL
+ * except start): hash_id[i], key_id[i] per particle; sorted_key[k],
+ * sorted_idx[k] per sorted position; start[key] for key < table_size
+ * (ORC_START_EMPTY when the bucket is empty).  Returns 0, or -1 on bad input. */
+int orc_stage_spatial(uint32_t n, uint32_t table_size, float h, const float *xp4,
+                      uint32_t *hash_id, uint32_t *key_id,
+                      uint32_t *sorted_key, uint32_t *sorted_idx, uint32_t *start);
+
+/* Stage C, cc:304-311 + 325-365.  dens2[2i] = density, dens2[2i+1] = near density. */
+void orc_stage_density(uint32_t n, uint32_t table_size, const orc_params *p,
+                       const float *xp4, const uint32_t *sorted_key,
+                       const uint32_t *sorted_idx, const uint32_t *hash_id,
+                       const uint32_t *start, float *dens2);
+
+/* Stage D, cc:62-69 + 367-422: vel4 += (F/rho)*dt. */
+void orc_stage_pressure(uint32_t n, uint32_t table_size, const orc_params *p, float dt,
+                        const float *xp4, const uint32_t *sorted_key,
+                        const uint32_t *sorted_idx, const uint32_t *hash_id,
+                        const uint32_t *start, const float *dens2, float *vel4);
+
+/* Stage E, cc:71-78 + 424-464.  gauss_seidel=1: in place, index order (the
+ * reference under a serial PSTL backend).  gauss_seidel=0: Jacobi. */
+void orc_stage_viscosity(uint32_t n, uint32_t table_size, const orc_params *p, float dt,
+                         const float *xp4, const uint32_t *sorted_key,
+                         const uint32_t *sorted_idx, const uint32_t *hash_id,
+                         const uint32_t *start, float *vel4, int gauss_seidel);
+
+/* Stage F, cc:80-108: pos += vel*dt, box collision; pos4[.w] = 0.34 (OutPositions). */
+void orc_stage_integrate(uint32_t n, const orc_params *p, float dt, float *pos4, float *vel4);
+
+/* One Update(dt) (cc:39-111).  dens2 receives the densities of this update
+ * (may be NULL).  stage_ms (may be NULL) receives the six stage wall times in
+ * ms, in the order of physicsWorld.h:45-50.  Returns 0 or -1 (bad input / OOM). */
+int orc_update(uint32_t n, uint32_t table_size, const orc_params *p, float dt,
+               float *pos4, float *vel4, float *dens2, int gauss_seidel, double *stage_ms);
+
+/* Neighbour sets as the reference's bucket walk sees them (cc:331-363):
+ * for particle i, every j reached through the 27 cells whose float-rounded
+ * hash matches and whose squared distance is <= r2, self included, in walk
+ * order.  CSR output: offs[n+1], list[offs[n]].  If list is NULL only offs is
+ * filled.  Returns total count. */
+size_t orc_neighbours(uint32_t n, uint32_t table_size, const orc_params *p,
+                      const float *xp4, const uint32_t *sorted_key,
+                      const uint32_t *sorted_idx, const uint32_t *hash_id,
+                      const uint32_t *start, uint64_t *offs, uint32_t *list);
+
+/* ---- FP64 restatement on the FP32 neighbour structure (for tolerances) ---- */
+/* Density in double from the same xp (as double). */
+void orc64_density(uint32_t n, uint32_t table_size, const orc_params *p,
+                   const float *xp4, const uint32_t *sorted_key,
+                   const uint32_t *sorted_idx, const uint32_t *hash_id,
+                   const uint32_t *start, double *dens2);
+/* Pressure acceleration*dt in double: dv3[3i+c] = (F_c/rho_i)*dt; mag3 =
+ * sum over pair terms of |term_c| / rho_i * dt (the scale of the sum). */
+void orc64_pressure(uint32_t n, uint32_t table_size, const orc_params *p, double dt,
+                    const float *xp4, const uint32_t *sorted_key,
+                    const uint32_t *sorted_idx, const uint32_t *hash_id,
+                    const uint32_t *start, const double *dens2,
+                    double *dv3, double *mag3);
+/* Jacobi viscosity in double: vin3 (n*3 doubles) -> dv3 = F*mu*dt, mag3 = sum |terms|*mu*dt. */
+void orc64_viscosity(uint32_t n, uint32_t table_size, const orc_params *p, double dt,
+                     const float *xp4, const uint32_t *sorted_key,
+                     const uint32_t *sorted_idx, const uint32_t *hash_id,
+                     const uint32_t *start, const double *vin3,
+                     double *dv3, double *mag3);
+
+/* ---- GridArrangement (cc:112-147, 518-557): the reference's initial lattice ---- */
+void orc_grid_arrangement(uint32_t n, float *pos4);
+
+/* OpenMP thread count for the per-particle loops (1 = serial). */
+void orc_set_threads(int nthreads);
+int orc_get_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
