@@ -58,6 +58,14 @@ def cpu_baseline(sc, substeps):
                       f"on 1 host thread, {el:.1f} s"}
 
 
+def copy_calibration(mib):
+    """A plain device copy kernel of known size (k_copy), so that per-dispatch
+    FETCH_SIZE / WRITE_SIZE counters can be scaled against known bytes."""
+    ms = S.copy_calibration(mib << 20, 3)
+    log(f"[bench] copy calibration: {mib} MiB copy in {ms:.4f} ms "
+        f"({2 * (mib << 20) / (ms * 1e-3) / 1e9:.0f} GB/s read+write)")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +75,9 @@ def main():
     ap.add_argument("--latency-steps", type=int, default=100)
     ap.add_argument("--cpu-substeps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="also run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -76,6 +87,8 @@ def main():
         print("no HIP device visible", file=sys.stderr)
         return 1
 
+    if args.copy_calibration_mib:
+        copy_calibration(args.copy_calibration_mib)
     sc = W.by_name(args.workload)
     log(f"[bench] workload {sc.name} n={sc.n} table={sc.table} steps={args.steps} warmup={args.warmup}")
     sim = S.Sim(sc.params, sc.n, sc.table)
@@ -84,6 +97,13 @@ def main():
         sim.step_async(sc.dt, sc.substeps)
     sim.sync()
     log("[bench] warmup done")
+    if not args.no_autotune:
+        sub_dt = float(np.float32(np.float32(sc.dt) / np.float32(sc.substeps)))
+        tuned = sim.autotune(sub_dt, updates=3)
+        log(f"[bench] autotune: {sim.variants} ms {np.round(tuned, 4)}")
+        for _ in range(2):
+            sim.step_async(sc.dt, sc.substeps)
+        sim.sync()
 
     # timed region: K steps, stage events recorded on the simulation stream
     sim.profile_begin(args.steps * sc.substeps)
@@ -104,6 +124,7 @@ def main():
         lat.append((time.perf_counter() - t) * 1e3)
     lat = np.array(lat)
     nonfinite = sim.count_nonfinite()
+    variants = sim.variants
     sim.close()
 
     value = sc.n * args.steps / elapsed
@@ -130,6 +151,7 @@ def main():
         "step_latency_ms": {"p50": float(np.percentile(lat, 50)),
                             "p99": float(np.percentile(lat, 99)), "n": int(lat.size)},
         "stage_ms_per_update": per,
+        "gather_variants": variants,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
                      "alg_bytes_per_particle": ALG_BYTES[dom]},

@@ -83,6 +83,39 @@ int sph_device_count(void)
     return n;
 }
 
+int sph_copy_calibration(uint64_t bytes, int reps, double *ms_per_copy)
+{
+    SPH_GUARD_BEGIN
+    if (bytes < 16 || reps < 1) return fail(SPH_ERR_INVALID, "bytes >= 16 and reps >= 1 required");
+    if (!have_gpu()) return fail(SPH_ERR_HIP, "no HIP device visible");
+    const size_t n = bytes / 16;
+    float4 *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc((void **)&a, n * 16);
+    if (e == hipSuccess) e = hipMalloc((void **)&b, n * 16);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMemsetAsync(a, 0, n * 16, st);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = sph::launch_copy(a, b, n, st);  // warm
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) e = sph::launch_copy(a, b, n, st);
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (ms_per_copy) *ms_per_copy = ms / reps;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (e != hipSuccess) return fail(SPH_ERR_HIP, std::string("copy calibration: ") + hipGetErrorString(e));
+    return SPH_OK;
+    SPH_GUARD_END
+}
+
 int sph_create(const sph_params *p, uint32_t n, uint32_t table_size, sph_handle *out)
 {
     SPH_GUARD_BEGIN
@@ -241,6 +274,47 @@ int sph_profile_end(sph_handle h, double mean_ms[6], uint32_t *updates)
     std::string err;
     return done(h->eng->profile_end(mean_ms, updates, &err), err);
     SPH_GUARD_END
+}
+
+int sph_autotune(sph_handle h, float dt, int updates, double ms[3])
+{
+    SPH_GUARD_BEGIN
+    SPH_NEED(h);
+    std::string err;
+    return done(h->eng->autotune(dt, updates, ms, &err), err);
+    SPH_GUARD_END
+}
+
+int sph_get_variants(sph_handle h, int32_t v[9])
+{
+    SPH_NEED(h);
+    if (!v) return fail(SPH_ERR_INVALID, "null output");
+    const sph::LaunchCfg &c = h->eng->launch_cfg();
+    const sph::GatherVariant g[3] = {c.density, c.pressure, c.viscosity};
+    for (int q = 0; q < 3; ++q) {
+        v[3 * q + 0] = g[q].work;
+        v[3 * q + 1] = g[q].list;
+        v[3 * q + 2] = g[q].block;
+    }
+    return SPH_OK;
+}
+
+int sph_set_variants(sph_handle h, const int32_t v[9])
+{
+    SPH_NEED(h);
+    if (!v) return fail(SPH_ERR_INVALID, "null input");
+    sph::LaunchCfg c = h->eng->launch_cfg();
+    sph::GatherVariant *g[3] = {&c.density, &c.pressure, &c.viscosity};
+    for (int q = 0; q < 3; ++q) {
+        const int b = v[3 * q + 2];
+        if (b != 64 && b != 128 && b != 192 && b != 256)
+            return fail(SPH_ERR_INVALID, "block must be 64, 128, 192 or 256");
+        g[q]->work = v[3 * q + 0] != 0;
+        g[q]->list = v[3 * q + 1] != 0;
+        g[q]->block = b;
+    }
+    h->eng->set_launch_cfg(c);
+    return SPH_OK;
 }
 
 int sph_count_nonfinite(sph_handle h, uint64_t *count)

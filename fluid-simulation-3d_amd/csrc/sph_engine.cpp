@@ -83,6 +83,19 @@ void fill_scene_desc(const sph_params &p, uint32_t n, uint32_t table, SceneDesc 
     d->lookahead = p.lookahead;                          // cc:47
     for (int c = 0; c < 3; ++c) d->half[c] = p.bounds[c] * 0.5f;  // cc:88
     d->damp_neg = -1 * p.damp;                           // cc:94
+    for (int c = 0; c < 3; ++c) d->cell_lo[c] = (int)floorf(-d->half[c] / r) - 4;
+}
+
+void work_key_layout(uint32_t n_scenes, EngineDesc *e)
+{
+    if (n_scenes <= 1) {
+        e->scene_shift = 32;
+        e->morton_drop = 0;
+        return;
+    }
+    const uint32_t sb = bits_for(n_scenes - 1);
+    e->scene_shift = 32 - sb;
+    e->morton_drop = e->scene_shift >= 30 ? 0 : 30 - e->scene_shift;
 }
 
 int Engine::check(hipError_t e, const char *what, std::string *err)
@@ -154,6 +167,7 @@ int Engine::create(const std::vector<SceneSpec> &scenes, uint32_t capacity,
     e->ed_h_->n_scenes = (uint32_t)scenes.size();
     e->key_bits_ = bits_for(table_capacity - 1);
     e->ed_h_->key_bits = e->key_bits_;
+    work_key_layout(e->ed_h_->n_scenes, e->ed_h_);
     e->d_.multi = scenes.size() > 1;
     rc = e->check(hipMemcpy(e->d_scene_of_, scene_of.data(), sizeof(uint32_t) * capacity,
                             hipMemcpyHostToDevice),
@@ -168,6 +182,12 @@ int Engine::create(const std::vector<SceneSpec> &scenes, uint32_t capacity,
     if (!rc)
         rc = e->check(hipMemsetAsync(e->d_.dens, 0, sizeof(float4) * capacity, e->stream_),
                       "clear dens", err);
+    // work order and rank hold valid indices (0) until the first rebuild
+    for (uint32_t *b : {e->d_.wvals[0], e->d_.wvals[1], e->d_.rank})
+        if (!rc)
+            rc = e->check(hipMemsetAsync(b, 0, sizeof(uint32_t) * capacity, e->stream_),
+                          "clear work order", err);
+    e->d_.work = e->d_.wvals[0];
     if (!rc) rc = e->check(hipStreamSynchronize(e->stream_), "sync", err);
     for (int i = 0; i < 7 && !rc; ++i) rc = e->check(hipEventCreate(&e->ev_[i]), "event", err);
     if (rc) {
@@ -210,9 +230,14 @@ int Engine::alloc(uint32_t cap, uint32_t tcap, std::string *err)
         rc = dm((void **)&d_.keys[b], u4, "keys");
         if (!rc) rc = dm((void **)&d_.vals[b], u4, "vals");
     }
-    if (!rc) rc = dm((void **)&d_.hist, sizeof(uint32_t) * 256 * (size_t)nblk, "radix hist");
+    if (!rc) rc = dm((void **)&d_.hist, sizeof(uint32_t) * 256 * ((size_t)nblk + 1), "radix hist");
     if (!rc) rc = dm((void **)&d_.start, sizeof(uint32_t) * (size_t)tcap, "start table");
     if (!rc) rc = dm((void **)&d_.end, sizeof(uint32_t) * (size_t)tcap, "end table");
+    for (int b = 0; b < 2 && !rc; ++b) {
+        rc = dm((void **)&d_.wkeys[b], u4, "work keys");
+        if (!rc) rc = dm((void **)&d_.wvals[b], u4, "work order");
+    }
+    if (!rc) rc = dm((void **)&d_.rank, u4, "rank");
     d_.ed = d_ed_;
     d_.sd = d_sd_;
     d_.scene_of = d_scene_of_;
@@ -226,7 +251,8 @@ Engine::~Engine()
     invalidate_graph();
     void *bufs[] = {d_ed_,       d_sd_,       d_scene_of_, d_.pos,     d_.vel,    d_.xp,
                     d_.xps,      d_.vs,       d_.vs2,      d_.dens,    d_.keys[0], d_.keys[1],
-                    d_.vals[0],  d_.vals[1],  d_.hist,     d_.start,   d_.end};
+                    d_.vals[0],  d_.vals[1],  d_.hist,     d_.start,   d_.end,
+                    d_.wkeys[0], d_.wkeys[1], d_.wvals[0], d_.wvals[1], d_.rank};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (ed_h_) (void)hipHostFree(ed_h_);
@@ -238,10 +264,12 @@ Engine::~Engine()
 
 void Engine::invalidate_graph()
 {
-    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
-    if (graph_) (void)hipGraphDestroy(graph_);
-    graph_exec_ = nullptr;
-    graph_ = nullptr;
+    for (int i = 0; i < 2; ++i) {
+        if (graph_exec_[i]) (void)hipGraphExecDestroy(graph_exec_[i]);
+        if (graph_[i]) (void)hipGraphDestroy(graph_[i]);
+        graph_exec_[i] = nullptr;
+        graph_[i] = nullptr;
+    }
 }
 
 int Engine::push_descs(std::string *err)
@@ -309,23 +337,29 @@ int Engine::upload(uint32_t s, const float *pos4, const float *vel4, std::string
     return SPH_OK;
 }
 
-hipError_t Engine::enqueue_update(hipEvent_t *ev)
+hipError_t Engine::enqueue_update(hipEvent_t *ev, bool rebuild_work)
 {
-    // ev (optional): 6 events bracketing predict | spatial | density |
-    // pressure | viscosity+integrate
+    // ev (optional): 6 events bracketing predict | spatial (+ work order) |
+    // density | pressure | viscosity+integrate
     hipError_t e;
 #define LK(x)                              \
     if ((e = (x)) != hipSuccess) return e;
     if (ev) LK(hipEventRecord(ev[0], stream_));
     LK(launch_predict(d_, cfg_, stream_));
     if (ev) LK(hipEventRecord(ev[1], stream_));
+    if (rebuild_work) {
+        int wb = 0;
+        LK(launch_radix_sort(d_, d_.wkeys, d_.wvals, 32, stream_, &wb));
+        d_.work = d_.wvals[wb];
+        work_valid_ = true;
+    }
     LK(launch_clear_table(d_, cfg_, stream_));
-    LK(launch_radix_sort(d_, key_bits_, stream_, &sorted_buf_));
+    LK(launch_radix_sort(d_, d_.keys, d_.vals, key_bits_, stream_, &sorted_buf_));
     LK(launch_reorder(d_, cfg_, sorted_buf_, stream_));
     if (ev) LK(hipEventRecord(ev[2], stream_));
-    LK(launch_density(d_, cfg_, stream_));
+    LK(launch_density(d_, cfg_, sorted_buf_, stream_));
     if (ev) LK(hipEventRecord(ev[3], stream_));
-    LK(launch_pressure(d_, cfg_, stream_));
+    LK(launch_pressure(d_, cfg_, sorted_buf_, stream_));
     if (ev) LK(hipEventRecord(ev[4], stream_));
     LK(launch_viscosity_integrate(d_, cfg_, sorted_buf_, stream_));
     if (ev) LK(hipEventRecord(ev[5], stream_));
@@ -341,11 +375,12 @@ int Engine::update(float dt, std::string *err)
     }
     int rc = set_dt(dt, err);
     if (rc) return rc;
+    const bool rebuild = rebuild_due();
     if (prof_active_ && prof_used_ * 6 < prof_ev_.size()) {
-        CK(enqueue_update(&prof_ev_[6 * prof_used_]), "launch profiled update");
+        CK(enqueue_update(&prof_ev_[6 * prof_used_], rebuild), "launch profiled update");
         ++prof_used_;
     } else if (timing_) {
-        CK(enqueue_update(ev_), "launch update");
+        CK(enqueue_update(ev_, rebuild), "launch update");
         CK(hipEventSynchronize(ev_[5]), "sync update");
         for (int i = 0; i < 5; ++i) {
             float ms = 0.0f;
@@ -354,20 +389,96 @@ int Engine::update(float dt, std::string *err)
         }
         stage_ms_[5] = 0.0;  // position + collision runs inside the viscosity kernel
     } else if (use_graph_) {
-        if (!graph_exec_) {
+        const int gi = rebuild ? 1 : 0;
+        if (!graph_exec_[gi]) {
             CK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture");
-            hipError_t le = enqueue_update(nullptr);
-            hipError_t ce = hipStreamEndCapture(stream_, &graph_);
+            hipError_t le = enqueue_update(nullptr, rebuild);
+            hipError_t ce = hipStreamEndCapture(stream_, &graph_[gi]);
             CK(le, "capture update");
             CK(ce, "end capture");
-            CK(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "instantiate graph");
+            CK(hipGraphInstantiate(&graph_exec_[gi], graph_[gi], nullptr, nullptr, 0),
+               "instantiate graph");
         }
-        CK(hipGraphLaunch(graph_exec_, stream_), "graph launch");
+        CK(hipGraphLaunch(graph_exec_[gi], stream_), "graph launch");
     } else {
-        CK(enqueue_update(nullptr), "launch update");
+        CK(enqueue_update(nullptr, rebuild), "launch update");
     }
+    ++work_updates_;
     have_tables_ = true;
     return SPH_OK;
+}
+
+int Engine::autotune(float dt, int updates, double ms_out[3], std::string *err)
+{
+    if (updates < 1 || updates > 64 || !isfinite(dt)) {
+        *err = "autotune needs finite dt and 1..64 updates per candidate";
+        return SPH_ERR_INVALID;
+    }
+    if (prof_active_) {
+        *err = "autotune while profiling";
+        return SPH_ERR_STATE;
+    }
+    const uint32_t n = ed_h_->n_total;
+    float4 *snap = nullptr;
+    CK(hipMalloc((void **)&snap, sizeof(float4) * 2 * (size_t)n), "autotune snapshot");
+    auto save = [&]() -> hipError_t {
+        hipError_t e = hipMemcpyAsync(snap, d_.pos, sizeof(float4) * n, hipMemcpyDeviceToDevice, stream_);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(snap + n, d_.vel, sizeof(float4) * n, hipMemcpyDeviceToDevice, stream_);
+        return e;
+    };
+    auto restore = [&]() -> hipError_t {
+        hipError_t e = hipMemcpyAsync(d_.pos, snap, sizeof(float4) * n, hipMemcpyDeviceToDevice, stream_);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_.vel, snap + n, sizeof(float4) * n, hipMemcpyDeviceToDevice, stream_);
+        return e;
+    };
+    const LaunchCfg keep = cfg_;
+    const bool keep_timing = timing_;
+    timing_ = false;
+    std::vector<GatherVariant> cands;
+    for (int w = 0; w < 2; ++w)
+        for (int l = 0; l < 2; ++l)
+            for (int b : {64, 128, 256}) cands.push_back(GatherVariant{w != 0, l != 0, b});
+    double best[3] = {1e30, 1e30, 1e30};
+    GatherVariant pick[3] = {keep.density, keep.pressure, keep.viscosity};
+    int rc = check(save(), "autotune snapshot", err);
+    for (size_t ci = 0; ci < cands.size() && !rc; ++ci) {
+        LaunchCfg c = keep;
+        c.density = c.pressure = c.viscosity = cands[ci];
+        cfg_ = c;
+        invalidate_graph();
+        work_valid_ = false;
+        // one untimed update (work order, caches), then the timed ones
+        rc = update(dt, err);
+        if (!rc) rc = profile_begin((uint32_t)updates, err);
+        for (int u = 0; u < updates && !rc; ++u) rc = update(dt, err);
+        double ms[6] = {0, 0, 0, 0, 0, 0};
+        uint32_t used = 0;
+        if (!rc) rc = profile_end(ms, &used, err);
+        if (!rc) rc = check(restore(), "autotune restore", err);
+        for (int q = 0; q < 3 && !rc; ++q)
+            if (ms[2 + q] < best[q]) {
+                best[q] = ms[2 + q];
+                pick[q] = cands[ci];
+            }
+    }
+    if (!rc) rc = check(hipStreamSynchronize(stream_), "autotune sync", err);
+    (void)hipFree(snap);
+    timing_ = keep_timing;
+    LaunchCfg chosen = keep;
+    if (!rc) {
+        chosen.density = pick[0];
+        chosen.pressure = pick[1];
+        chosen.viscosity = pick[2];
+    }
+    cfg_ = chosen;
+    invalidate_graph();
+    work_valid_ = false;
+    have_tables_ = false;
+    if (ms_out)
+        for (int q = 0; q < 3; ++q) ms_out[q] = best[q];
+    return rc;
 }
 
 int Engine::profile_begin(uint32_t max_updates, std::string *err)
@@ -444,10 +555,16 @@ int Engine::prime(std::string *err)
     int rc = set_dt(0.0f, err);
     if (rc) return rc;
     CK(launch_predict(d_, cfg_, stream_), "prime predict");
+    if (cfg_.needs_work()) {
+        int wb = 0;
+        CK(launch_radix_sort(d_, d_.wkeys, d_.wvals, 32, stream_, &wb), "prime work order");
+        d_.work = d_.wvals[wb];
+        work_valid_ = true;
+    }
     CK(launch_clear_table(d_, cfg_, stream_), "prime clear");
-    CK(launch_radix_sort(d_, key_bits_, stream_, &sorted_buf_), "prime sort");
+    CK(launch_radix_sort(d_, d_.keys, d_.vals, key_bits_, stream_, &sorted_buf_), "prime sort");
     CK(launch_reorder(d_, cfg_, sorted_buf_, stream_), "prime reorder");
-    CK(launch_density(d_, cfg_, stream_), "prime density");
+    CK(launch_density(d_, cfg_, sorted_buf_, stream_), "prime density");
     CK(hipStreamSynchronize(stream_), "prime sync");
     have_tables_ = true;
     return SPH_OK;

@@ -66,20 +66,37 @@ class Engine {
     // stage_ms), their count, and the pool is released.
     int profile_end(double mean_ms[6], uint32_t *updates, std::string *err);
     const double *stage_ms() const { return stage_ms_; }
-    LaunchCfg &launch_cfg() { return cfg_; }
+    const LaunchCfg &launch_cfg() const { return cfg_; }
+    void set_launch_cfg(const LaunchCfg &c)
+    {
+        cfg_ = c;
+        invalidate_graph();
+    }
+    // Times every gather-kernel variant on the current state (`updates`
+    // Update(dt) calls per candidate, from a snapshot of positions and
+    // velocities that is restored afterwards) and keeps the fastest variant of
+    // each kernel.  ms_out (may be null) gets the chosen variants' mean ms of
+    // density, pressure, viscosity.  Densities and the debug tables are
+    // recomputed by the next update.
+    int autotune(float dt, int updates, double ms_out[3], std::string *err);
     void invalidate_graph();
     int set_use_graph(bool on) { use_graph_ = on; invalidate_graph(); return 0; }
     // Mean duration (ms) of each gather kernel over the last timed update.
     uint64_t device_bytes() const { return device_bytes_; }
     const DevState &dev() const { return d_; }
     int sorted_buf() const { return sorted_buf_; }
+    void set_work_period(uint32_t p) { work_period_ = p ? p : 1; }
 
   private:
     Engine() = default;
     int alloc(uint32_t capacity, uint32_t table_capacity, std::string *err);
     int push_descs(std::string *err);
     int set_dt(float dt, std::string *err);
-    hipError_t enqueue_update(hipEvent_t *ev);
+    hipError_t enqueue_update(hipEvent_t *ev, bool rebuild_work);
+    bool rebuild_due() const
+    {
+        return cfg_.needs_work() && (!work_valid_ || work_updates_ % work_period_ == 0);
+    }
     int check(hipError_t e, const char *what, std::string *err);
 
     hipStream_t stream_ = nullptr;
@@ -97,8 +114,11 @@ class Engine {
     int sorted_buf_ = 0;
     bool have_tables_ = false;
     bool use_graph_ = true;
-    hipGraph_t graph_ = nullptr;
-    hipGraphExec_t graph_exec_ = nullptr;
+    hipGraph_t graph_[2] = {nullptr, nullptr};          // [rebuild work order?]
+    hipGraphExec_t graph_exec_[2] = {nullptr, nullptr};
+    uint32_t work_period_ = 8;   // updates between work-order rebuilds
+    uint64_t work_updates_ = 0;  // updates since creation
+    bool work_valid_ = false;    // a work order has been built
     bool timing_ = false;
     hipEvent_t ev_[7] = {};
     double stage_ms_[6] = {0, 0, 0, 0, 0, 0};

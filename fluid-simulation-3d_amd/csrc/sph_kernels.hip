@@ -37,7 +37,6 @@ constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
 constexpr int kSortBlock = 256;  // 4 waves
 constexpr int kSortWaves = kSortBlock / 64;
-constexpr int kScanBlock = 1024;
 
 __device__ __forceinline__ uint32_t grid_stride() { return gridDim.x * blockDim.x; }
 __device__ __forceinline__ uint32_t global_id() { return blockIdx.x * blockDim.x + threadIdx.x; }
@@ -55,15 +54,44 @@ __device__ __forceinline__ int cell_coord(float x, float h) { return (int)floorf
 // ---------------------------------------------------------------------------
 // Stage A + B1
 // ---------------------------------------------------------------------------
+// 10-bit Morton spread: bit i of x moves to bit 3i.
+__device__ __forceinline__ uint32_t spread3(uint32_t x)
+{
+    x &= 0x3ffu;
+    x = (x | (x << 16)) & 0x030000ffu;
+    x = (x | (x << 8)) & 0x0300f00fu;
+    x = (x | (x << 4)) & 0x030c30c3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+// Ordering key of the gather kernels' work order: scene index in the top
+// bits, then the Morton code of the cell (relative to the scene's box,
+// clamped to 10 bits per axis).  It only decides WHICH work-item handles a
+// particle, never a result, so clamping and truncation are harmless.
+__device__ __forceinline__ uint32_t work_key(const EngineDesc &E, const SceneDesc &S, uint32_t s,
+                                             int cx, int cy, int cz)
+{
+    const uint32_t ux = (uint32_t)min(max(cx - S.cell_lo[0], 0), 1023);
+    const uint32_t uy = (uint32_t)min(max(cy - S.cell_lo[1], 0), 1023);
+    const uint32_t uz = (uint32_t)min(max(cz - S.cell_lo[2], 0), 1023);
+    const uint32_t m = spread3(ux) | (spread3(uy) << 1) | (spread3(uz) << 2);
+    return E.scene_shift >= 32 ? m : ((s << E.scene_shift) | (m >> E.morton_drop));
+}
+
+// ---------------------------------------------------------------------------
+// Stage A + B1
+// ---------------------------------------------------------------------------
 template <bool MULTI>
 __global__ void k_predict(const EngineDesc *__restrict__ ed, const SceneDesc *__restrict__ sd,
                           const uint32_t *__restrict__ scene_of, const float4 *__restrict__ pos,
                           float4 *__restrict__ vel, float4 *__restrict__ xp,
-                          uint32_t *__restrict__ keys, uint32_t *__restrict__ vals)
+                          uint32_t *__restrict__ keys, uint32_t *__restrict__ wkeys)
 {
-    const uint32_t n = ed->n_total;
-    for (uint32_t g = global_id(); g < n; g += grid_stride()) {
-        const SceneDesc &S = scene_of_slot<MULTI>(sd, scene_of, g);
+    const EngineDesc E = *ed;
+    for (uint32_t g = global_id(); g < E.n_total; g += grid_stride()) {
+        const uint32_t s = MULTI ? scene_of[g] : 0u;
+        const SceneDesc &S = sd[s];
         const float dt = S.dt;
         float4 v = vel[g];
         const float4 x = pos[g];
@@ -75,12 +103,12 @@ __global__ void k_predict(const EngineDesc *__restrict__ ed, const SceneDesc *__
         const float px = x.x + v.x * S.lookahead;
         const float py = x.y + v.y * S.lookahead;
         const float pz = x.z + v.z * S.lookahead;
-        const uint32_t hash =
-            hash_cell(cell_coord(px, S.h), cell_coord(py, S.h), cell_coord(pz, S.h));
+        const int cx = cell_coord(px, S.h), cy = cell_coord(py, S.h), cz = cell_coord(pz, S.h);
+        const uint32_t hash = hash_cell(cx, cy, cz);
         vel[g] = v;
         xp[g] = make_float4(px, py, pz, __uint_as_float(hash));
         keys[g] = S.key_off + fastmod(hash, S.mod_m, S.table);
-        vals[g] = g;
+        wkeys[g] = work_key(E, S, s, cx, cy, cz);
     }
 }
 
@@ -125,68 +153,72 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_hist(const EngineDesc *__r
     }
 }
 
-// Exclusive scan of `m` counters in place, one block of 1024 work-items,
-// 8 consecutive counters per work-item per round.
-constexpr int kScanItems = 8;
-__global__ __launch_bounds__(kScanBlock) void k_scan_exclusive(uint32_t *__restrict__ a, uint32_t m)
+// Block-wide exclusive scan of one value per work-item (kSortBlock items);
+// returns the exclusive prefix and writes the block total to *total.
+__device__ __forceinline__ uint32_t block_scan_excl(uint32_t v, uint32_t *wsum, uint32_t *total)
 {
-    __shared__ uint32_t wsum[kScanBlock / 64];
-    __shared__ uint32_t carry_s;
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    if (tid == 0) carry_s = 0;
-    __syncthreads();
-    for (uint32_t base = 0; base < m; base += kScanBlock * kScanItems) {
-        const uint32_t i0 = base + tid * kScanItems;
-        uint32_t v[kScanItems];
-        uint32_t tsum = 0;
+    uint32_t x = v;
 #pragma unroll
-        for (int j = 0; j < kScanItems; ++j) {
-            v[j] = i0 + j < m ? a[i0 + j] : 0u;
-            tsum += v[j];
-        }
-        // inclusive wave scan of the per-item sums
-        uint32_t x = tsum;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t y = __shfl_up(x, off, 64);
-            if (lane >= (uint32_t)off) x += y;
-        }
-        if (lane == 63) wsum[w] = x;
-        __syncthreads();
-        if (w == 0) {
-            uint32_t s = lane < kScanBlock / 64 ? wsum[lane] : 0;
-#pragma unroll
-            for (int off = 1; off < kScanBlock / 64; off <<= 1) {
-                const uint32_t y = __shfl_up(s, off, 64);
-                if (lane >= (uint32_t)off) s += y;
-            }
-            if (lane < kScanBlock / 64) wsum[lane] = s;  // inclusive over waves
-        }
-        __syncthreads();
-        const uint32_t carry = carry_s;
-        uint32_t run = carry + (w ? wsum[w - 1] : 0) + x - tsum;
-#pragma unroll
-        for (int j = 0; j < kScanItems; ++j) {
-            if (i0 + j < m) a[i0 + j] = run;
-            run += v[j];
-        }
-        __syncthreads();
-        if (tid == kScanBlock - 1) carry_s = carry + wsum[kScanBlock / 64 - 1];
-        __syncthreads();
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= (uint32_t)off) x += y;
     }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (int q = 0; q < kSortWaves; ++q) {
+        before += (uint32_t)q < w ? wsum[q] : 0u;
+        all += wsum[q];
+    }
+    *total = all;
+    __syncthreads();
+    return before + x - v;
+}
+
+// One block per digit: exclusive scan of that digit's per-block counts (in
+// place) and the digit's total count.
+__global__ __launch_bounds__(kSortBlock) void k_radix_scan_digit(uint32_t *__restrict__ hist,
+                                                                  uint32_t nblk,
+                                                                  uint32_t *__restrict__ digit_total)
+{
+    __shared__ uint32_t wsum[kSortWaves];
+    const uint32_t d = blockIdx.x, tid = threadIdx.x;
+    uint32_t *row = hist + (size_t)d * nblk;
+    const uint32_t per = (nblk + kSortBlock - 1) / kSortBlock;  // <= 4 for nblk <= 1024
+    uint32_t v[4] = {0, 0, 0, 0}, sum = 0;
+    for (uint32_t j = 0; j < per && j < 4; ++j) {
+        const uint32_t i = tid * per + j;
+        v[j] = i < nblk ? row[i] : 0u;
+        sum += v[j];
+    }
+    uint32_t total;
+    uint32_t run = block_scan_excl(sum, wsum, &total);
+    for (uint32_t j = 0; j < per && j < 4; ++j) {
+        const uint32_t i = tid * per + j;
+        if (i < nblk) row[i] = run;
+        run += v[j];
+    }
+    if (tid == 0) digit_total[d] = total;
 }
 
 __global__ __launch_bounds__(kSortBlock) void k_radix_scatter(
     const EngineDesc *__restrict__ ed, const uint32_t *__restrict__ kin,
     const uint32_t *__restrict__ vin, uint32_t *__restrict__ kout, uint32_t *__restrict__ vout,
-    uint32_t shift, const uint32_t *__restrict__ hist)
+    uint32_t shift, const uint32_t *__restrict__ hist, const uint32_t *__restrict__ digit_total)
 {
+    static_assert(kRadix == kSortBlock, "one digit per work-item");
     __shared__ uint32_t base[kRadix];
     __shared__ uint32_t wcnt[kSortWaves][kRadix];
+    __shared__ uint32_t wsum[kSortWaves];
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    for (uint32_t d = tid; d < kRadix; d += kSortBlock) {
-        base[d] = hist[d * gridDim.x + blockIdx.x];
-        for (int q = 0; q < kSortWaves; ++q) wcnt[q][d] = 0;
+    {
+        // global offset of digit `tid` = (counts of smaller digits) + (this
+        // digit's counts in earlier blocks)
+        uint32_t total;
+        const uint32_t digit_base = block_scan_excl(digit_total[tid], wsum, &total);
+        base[tid] = digit_base + hist[tid * gridDim.x + blockIdx.x];
+        for (int q = 0; q < kSortWaves; ++q) wcnt[q][tid] = 0;
     }
     __syncthreads();
     uint32_t beg, end;
@@ -196,7 +228,7 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_scatter(
         const uint32_t e = t0 + tid;
         const bool valid = e < end;
         const uint32_t key = valid ? kin[e] : 0u;
-        const uint32_t val = valid ? vin[e] : 0u;
+        const uint32_t val = valid ? (vin ? vin[e] : e) : 0u;  // no input values: slot index
         const uint32_t d = (key >> shift) & (kRadix - 1);
         // lanes of this wave holding the same digit: intersect one ballot per bit
         uint64_t peers = __ballot(valid);
@@ -235,7 +267,7 @@ __global__ void k_reorder(const EngineDesc *__restrict__ ed, const uint32_t *__r
                           const uint32_t *__restrict__ svals, const float4 *__restrict__ xp,
                           const float4 *__restrict__ vel, float4 *__restrict__ xps,
                           float4 *__restrict__ vs, uint32_t *__restrict__ start,
-                          uint32_t *__restrict__ endt)
+                          uint32_t *__restrict__ endt, uint32_t *__restrict__ rank)
 {
     const uint32_t n = ed->n_total;
     for (uint32_t k = global_id(); k < n; k += grid_stride()) {
@@ -245,6 +277,7 @@ __global__ void k_reorder(const EngineDesc *__restrict__ ed, const uint32_t *__r
         // the reference's lookup record keeps the hash as a float (cc:480)
         xps[k] = make_float4(r.x, r.y, r.z, (float)__float_as_uint(r.w));
         vs[k] = vel[g];
+        rank[g] = k;
         const uint32_t prev = k ? skeys[k - 1] : 0xffffffffu;
         const uint32_t next = k + 1 < n ? skeys[k + 1] : 0xffffffffu;
         if (key != prev) start[key] = k;
@@ -256,155 +289,278 @@ __global__ void k_reorder(const EngineDesc *__restrict__ ed, const uint32_t *__r
 // Neighbour walk (cc:331-363, 378-419, 432-462): 27 cells in the reference's
 // offset order (physicsWorld.h:131-143: dx outer, dz inner); within a bucket
 // in sorted order; records of other cells sharing the bucket are skipped by
-// the float-rounded hash test.  `body(m, ox, oy, oz, d2)` runs for every
-// record with d2 <= r2 (the sqrRadius prefilter) and d < h; records with
-// d >= h contribute exactly zero to every sum of the reference.
+// the float-rounded hash test.  `body(m, ox, oy, oz, d2, aux)` runs for every
+// record with d2 <= r2 (the sqrRadius prefilter) and d < h, in that order;
+// records with d >= h contribute exactly zero to every sum of the reference.
+//
+// Latency hiding without changing the summation order: the bucket range of
+// the next cell is fetched while the current bucket is walked, and a bucket is
+// walked KB records at a time (all loads of a batch issued before any test).
+// AUX: also fetch aux[m] (the neighbour's density or velocity) for the
+// records of a batch that pass the tests.
 // ---------------------------------------------------------------------------
-template <bool SKIP_SELF, typename Body>
+constexpr int KB = 4;
+
+template <bool SKIP_SELF, bool AUX, typename Body>
 __device__ __forceinline__ void walk(const SceneDesc &S, const float4 *__restrict__ xps,
+                                     const float4 *__restrict__ aux,
                                      const uint32_t *__restrict__ start,
                                      const uint32_t *__restrict__ endt, uint32_t k,
                                      const float4 &pi, Body &&body)
 {
     const int cx = cell_coord(pi.x, S.h), cy = cell_coord(pi.y, S.h), cz = cell_coord(pi.z, S.h);
+    const float r2 = S.r2, h2lt = S.h2_lt;
+    const uint32_t key_off = S.key_off, table = S.table;
+    const uint64_t mod_m = S.mod_m;
+    uint32_t nb, ne;
+    float nhf;
+    auto fetch = [&](int o) {
+        const int dx = o / 9 - 1, dy = (o / 3) % 3 - 1, dz = o % 3 - 1;
+        const uint32_t hash = hash_cell(cx + dx, cy + dy, cz + dz);
+        const uint32_t key = key_off + fastmod(hash, mod_m, table);
+        nb = start[key];
+        ne = endt[key];  // meaningful only when nb != SPH_START_EMPTY
+        nhf = (float)hash;
+    };
+    fetch(0);
 #pragma unroll 1
-    for (int dx = -1; dx <= 1; ++dx)
+    for (int o = 0; o < 27; ++o) {
+        const uint32_t b = nb, e = ne;
+        const float hf = nhf;
+        if (o + 1 < 27) fetch(o + 1);
+        if (b == SPH_START_EMPTY) continue;
 #pragma unroll 1
-        for (int dy = -1; dy <= 1; ++dy)
-#pragma unroll 1
-            for (int dz = -1; dz <= 1; ++dz) {
-                const uint32_t hash = hash_cell(cx + dx, cy + dy, cz + dz);
-                const uint32_t key = S.key_off + fastmod(hash, S.mod_m, S.table);
-                const uint32_t b = start[key];
-                if (b == SPH_START_EMPTY) continue;
-                const uint32_t e = endt[key];
-                const float hf = (float)hash;
-                for (uint32_t m = b; m < e; ++m) {
-                    const float4 r = xps[m];
-                    if (r.w != hf) continue;
-                    if (SKIP_SELF && m == k) continue;
-                    const float ox = r.x - pi.x, oy = r.y - pi.y, oz = r.z - pi.z;
-                    const float d2 = ox * ox + oy * oy + oz * oz;  // glm::dot order
-                    if (d2 > S.r2) continue;
-                    if (!(d2 < S.h2_lt)) continue;
-                    body(m, ox, oy, oz, d2);
-                }
+        for (uint32_t m = b; m < e; m += KB) {
+            float4 r[KB];
+#pragma unroll
+            for (int j = 0; j < KB; ++j) {
+                r[j] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(0x7fc00000));  // NaN: no match
+                if (m + j < e) r[j] = xps[m + j];
             }
-}
-
-template <bool MULTI>
-__global__ void k_density(const EngineDesc *__restrict__ ed, const SceneDesc *__restrict__ sd,
-                          const uint32_t *__restrict__ scene_of, const float4 *__restrict__ xps,
-                          const uint32_t *__restrict__ start, const uint32_t *__restrict__ endt,
-                          float4 *__restrict__ dens)
-{
-    const uint32_t n = ed->n_total;
-    for (uint32_t k = global_id(); k < n; k += grid_stride()) {
-        const SceneDesc &S = scene_of_slot<MULTI>(sd, scene_of, k);
-        const float4 pi = xps[k];
-        const float h = S.h, w2c = S.w2c, w3c = S.w3c;
-        float rho = 0.0f, rhon = 0.0f;
-        walk<false>(S, xps, start, endt, k, pi, [&](uint32_t, float, float, float, float d2) {
-            const float d = sqrtf(d2);
-            const float v = h - d;
-            rho += v * v * w2c;       // SmoothingPow2, kernels.h:31
-            rhon += v * v * v * w3c;  // SmoothingPow3, kernels.h:43
-        });
-        // pressures of cc:371-372 / 407-408, computed once per particle
-        dens[k] = make_float4(rho, rhon, (rho - S.rho0) * S.k, rhon * S.k_near);
-    }
-}
-
-template <bool MULTI>
-__global__ void k_pressure(const EngineDesc *__restrict__ ed, const SceneDesc *__restrict__ sd,
-                           const uint32_t *__restrict__ scene_of, const float4 *__restrict__ xps,
-                           const uint32_t *__restrict__ start, const uint32_t *__restrict__ endt,
-                           const float4 *__restrict__ dens, const float4 *__restrict__ vs,
-                           float4 *__restrict__ vs2)
-{
-    const uint32_t n = ed->n_total;
-    for (uint32_t k = global_id(); k < n; k += grid_stride()) {
-        const SceneDesc &S = scene_of_slot<MULTI>(sd, scene_of, k);
-        const float4 pi = xps[k];
-        const float4 di = dens[k];
-        const float h = S.h, dw2c = S.dw2c, dw3c = S.dw3c;
-        const float P = di.z, Pn = di.w;
-        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-        walk<true>(S, xps, start, endt, k, pi, [&](uint32_t m, float ox, float oy, float oz, float d2) {
-            const float4 dj = dens[m];
-            const float sp = (P + dj.z) * 0.5f;   // cc:410
-            const float snp = (Pn + dj.w) * 0.5f; // cc:411
-            const float d = sqrtf(d2);
-            float dirx = 0.0f, diry = 1.0f, dirz = 0.0f;  // cc:414
-            if (d > 0.0f) {
-                dirx = ox / d;
-                diry = oy / d;
-                dirz = oz / d;
+            bool pass[KB];
+            float ox[KB], oy[KB], oz[KB], d2[KB];
+#pragma unroll
+            for (int j = 0; j < KB; ++j) {
+                ox[j] = r[j].x - pi.x;
+                oy[j] = r[j].y - pi.y;
+                oz[j] = r[j].z - pi.z;
+                d2[j] = ox[j] * ox[j] + oy[j] * oy[j] + oz[j] * oz[j];  // glm::dot order
+                pass[j] = r[j].w == hf && !(SKIP_SELF && m + j == k) && !(d2[j] > r2) &&
+                          d2[j] < h2lt;
             }
-            const float v = h - d;
-            const float a = -v * dw2c;      // SmoothingDerivativePow2, kernels.h:55
-            fx += dirx * a * sp / dj.x;     // cc:416
-            fy += diry * a * sp / dj.x;
-            fz += dirz * a * sp / dj.x;
-            const float b = -v * v * dw3c;  // SmoothingDerivativePow3, kernels.h:67
-            fx += dirx * b * snp / dj.y;    // cc:417
-            fy += diry * b * snp / dj.y;
-            fz += dirz * b * snp / dj.y;
-        });
-        float4 v = vs[k];
-        const float dt = S.dt, rho = di.x;
-        v.x = v.x + (fx / rho) * dt;  // cc:421
-        v.y = v.y + (fy / rho) * dt;
-        v.z = v.z + (fz / rho) * dt;
-        vs2[k] = v;
+            float4 a[KB];
+#pragma unroll
+            for (int j = 0; j < KB; ++j) {
+                a[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (AUX && pass[j]) a[j] = aux[m + j];
+            }
+#pragma unroll
+            for (int j = 0; j < KB; ++j)
+                if (pass[j]) body(m + j, ox[j], oy[j], oz[j], d2[j], a[j]);
+        }
     }
 }
 
-template <bool MULTI>
-__global__ void k_viscosity_integrate(const EngineDesc *__restrict__ ed,
-                                      const SceneDesc *__restrict__ sd,
-                                      const uint32_t *__restrict__ scene_of,
-                                      const float4 *__restrict__ xps,
-                                      const uint32_t *__restrict__ start,
-                                      const uint32_t *__restrict__ endt,
-                                      const float4 *__restrict__ vs2,
-                                      const uint32_t *__restrict__ svals,
-                                      float4 *__restrict__ pos, float4 *__restrict__ vel)
+// Gather kernels (density, pressure, viscosity), in result-identical
+// variants picked per workload by the autotuner (sph_engine.cpp):
+//
+// WORK: work-item i handles particle work[i] (ids sorted by the Morton code
+//   of their cell, rebuilt every few updates) at sorted slot rank[id], so
+//   neighbouring work-items own neighbouring cells and read largely the same
+//   buckets, which stay in L2; workgroups are renumbered so each XCD takes one
+//   contiguous run of the work order.  Without WORK, work-item i handles
+//   sorted slot i (cells in hash order).
+// LIST: two phases per particle -- the walk only tests candidates and
+//   records the passing ones in a per-lane list in LDS (kListMax entries);
+//   the arithmetic then runs in a dense loop over the list, so lanes do not
+//   idle through the heavy body while other lanes test candidates.  The list
+//   keeps walk order (sums keep the reference's order); a particle with more
+//   than kListMax neighbours finishes with a second walk that skips the
+//   listed ones.  Without LIST the body runs inside the walk.
+// Which work-item handles a particle, and whether the body runs in the walk
+// or from the list, never changes a result.
+constexpr uint32_t kListMax = 56;
+
+template <bool WORK>
+__device__ __forceinline__ uint32_t gather_index()
 {
-    const uint32_t n = ed->n_total;
-    for (uint32_t k = global_id(); k < n; k += grid_stride()) {
-        const SceneDesc &S = scene_of_slot<MULTI>(sd, scene_of, k);
-        const float4 pi = xps[k];
-        const float4 vi = vs2[k];
-        const float hh = S.hh, c6 = S.poly6c;
-        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-        walk<true>(S, xps, start, endt, k, pi, [&](uint32_t m, float, float, float, float d2) {
-            const float d = sqrtf(d2);
-            const float v = hh - d * d;          // kernels.h:78
-            const float w = v * v * v * c6;      // kernels.h:79
-            const float4 vj = vs2[m];
-            fx += (vj.x - vi.x) * w;             // cc:460
-            fy += (vj.y - vi.y) * w;
-            fz += (vj.z - vi.z) * w;
-        });
-        const float dt = S.dt, mu = S.mu;
-        float4 v = vi;
-        v.x = vi.x + fx * mu * dt;  // cc:463
-        v.y = vi.y + fy * mu * dt;
-        v.z = vi.z + fz * mu * dt;
-        // stage F (cc:84-107), per particle
-        const uint32_t g = svals[k];
-        float4 x = pos[g];
-        x.x += v.x * dt;
-        x.y += v.y * dt;
-        x.z += v.z * dt;
-        if (S.half[0] - fabsf(x.x) <= 0.0f) { x.x = S.half[0] * sign_f(x.x); v.x *= S.damp_neg; }
-        if (S.half[1] - fabsf(x.y) <= 0.0f) { x.y = S.half[1] * sign_f(x.y); v.y *= S.damp_neg; }
-        if (S.half[2] - fabsf(x.z) <= 0.0f) { x.z = S.half[2] * sign_f(x.z); v.z *= S.damp_neg; }
-        x.w = 0.34f;  // OutPositions[i] = vec4(positions[i], 0.34f), cc:107
-        pos[g] = x;
-        vel[g] = v;
+    if (!WORK) return blockIdx.x * blockDim.x + threadIdx.x;
+    // workgroups are dealt round-robin to the 8 XCDs (block b -> XCD b % 8);
+    // the grid is a multiple of 8
+    const uint32_t G = gridDim.x, b = blockIdx.x;
+    const uint32_t lb = (b & 7u) * (G >> 3) + (b >> 3);
+    return lb * blockDim.x + threadIdx.x;
+}
+
+// Runs body(m, ox, oy, oz, d2, aux) over the neighbours of slot k with d < h
+// (self excluded when SKIP_SELF), in walk order.  With LIST the candidate
+// tests and the bodies are split into the two phases described above; `lst`
+// is this work-item's column of the block's LDS list (stride blockDim.x).
+template <bool LIST, bool SKIP_SELF, bool AUX, typename Body>
+__device__ __forceinline__ void gather(const SceneDesc &S, const float4 *__restrict__ xps,
+                                       const float4 *__restrict__ aux,
+                                       const uint32_t *__restrict__ start,
+                                       const uint32_t *__restrict__ endt, uint32_t *lst,
+                                       uint32_t k, const float4 &pi, Body &&body)
+{
+    if (!LIST) {
+        walk<SKIP_SELF, AUX>(S, xps, aux, start, endt, k, pi, body);
+        return;
     }
+    const uint32_t stride = blockDim.x;
+    uint32_t c = 0;
+    walk<SKIP_SELF, false>(S, xps, nullptr, start, endt, k, pi,
+                           [&](uint32_t m, float, float, float, float, const float4 &) {
+                               if (c < kListMax) lst[c * stride] = m;
+                               ++c;
+                           });
+    const uint32_t n1 = min(c, kListMax);
+    for (uint32_t j = 0; j < n1; j += KB) {
+        uint32_t m[KB];
+        float4 r[KB], a[KB];
+#pragma unroll
+        for (int q = 0; q < KB; ++q) {
+            m[q] = k;
+            if (j + q < n1) m[q] = lst[(j + q) * stride];
+        }
+#pragma unroll
+        for (int q = 0; q < KB; ++q) {
+            r[q] = pi;
+            a[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (j + q < n1) {
+                r[q] = xps[m[q]];
+                if (AUX) a[q] = aux[m[q]];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < KB; ++q)
+            if (j + q < n1) {
+                const float ox = r[q].x - pi.x, oy = r[q].y - pi.y, oz = r[q].z - pi.z;
+                body(m[q], ox, oy, oz, ox * ox + oy * oy + oz * oz, a[q]);
+            }
+    }
+    if (c > kListMax) {
+        uint32_t skip = kListMax;
+        walk<SKIP_SELF, AUX>(S, xps, aux, start, endt, k, pi,
+                             [&](uint32_t m, float ox, float oy, float oz, float d2,
+                                 const float4 &a) {
+                                 if (skip) --skip;
+                                 else body(m, ox, oy, oz, d2, a);
+                             });
+    }
+}
+
+#define SPH_GATHER_PROLOGUE()                                                       \
+    extern __shared__ uint32_t lds_list[];                                          \
+    const uint32_t i = gather_index<WORK>();                                        \
+    if (i >= ed->n_total) return;                                                   \
+    const uint32_t g = WORK ? work[i] : svals[i];                                   \
+    const uint32_t k = WORK ? rank[g] : i;                                          \
+    const SceneDesc &S = scene_of_slot<MULTI>(sd, scene_of, k);                     \
+    const float4 pi = xps[k];                                                       \
+    uint32_t *lst = lds_list + threadIdx.x;                                         \
+    (void)g;                                                                        \
+    (void)lst
+
+#define SPH_GATHER_PARAMS                                                                     \
+    const EngineDesc *__restrict__ ed, const SceneDesc *__restrict__ sd,                      \
+        const uint32_t *__restrict__ scene_of, const uint32_t *__restrict__ work,             \
+        const uint32_t *__restrict__ rank, const uint32_t *__restrict__ svals,                \
+        const float4 *__restrict__ xps, const uint32_t *__restrict__ start,                   \
+        const uint32_t *__restrict__ endt
+
+template <bool MULTI, bool WORK, bool LIST>
+__global__ __launch_bounds__(256) void k_density(SPH_GATHER_PARAMS, float4 *__restrict__ dens)
+{
+    SPH_GATHER_PROLOGUE();
+    const float h = S.h, w2c = S.w2c, w3c = S.w3c;
+    float rho = 0.0f, rhon = 0.0f;
+    gather<LIST, false, false>(S, xps, nullptr, start, endt, lst, k, pi,
+                               [&](uint32_t, float, float, float, float d2, const float4 &) {
+        const float d = sqrtf(d2);
+        const float v = h - d;
+        rho += v * v * w2c;       // SmoothingPow2, kernels.h:31
+        rhon += v * v * v * w3c;  // SmoothingPow3, kernels.h:43
+    });
+    // pressures of cc:371-372 / 407-408, computed once per particle
+    dens[k] = make_float4(rho, rhon, (rho - S.rho0) * S.k, rhon * S.k_near);
+}
+
+template <bool MULTI, bool WORK, bool LIST>
+__global__ __launch_bounds__(256) void k_pressure(SPH_GATHER_PARAMS, const float4 *__restrict__ dens,
+                                                   const float4 *__restrict__ vs,
+                                                   float4 *__restrict__ vs2)
+{
+    SPH_GATHER_PROLOGUE();
+    const float4 di = dens[k];
+    const float h = S.h, dw2c = S.dw2c, dw3c = S.dw3c;
+    const float P = di.z, Pn = di.w;
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    gather<LIST, true, true>(S, xps, dens, start, endt, lst, k, pi,
+                             [&](uint32_t, float ox, float oy, float oz, float d2,
+                                 const float4 &dj) {
+        const float sp = (P + dj.z) * 0.5f;   // cc:410
+        const float snp = (Pn + dj.w) * 0.5f; // cc:411
+        const float d = sqrtf(d2);
+        float dirx = 0.0f, diry = 1.0f, dirz = 0.0f;  // cc:414
+        if (d > 0.0f) {
+            dirx = ox / d;
+            diry = oy / d;
+            dirz = oz / d;
+        }
+        const float v = h - d;
+        const float a = -v * dw2c;      // SmoothingDerivativePow2, kernels.h:55
+        fx += dirx * a * sp / dj.x;     // cc:416
+        fy += diry * a * sp / dj.x;
+        fz += dirz * a * sp / dj.x;
+        const float b = -v * v * dw3c;  // SmoothingDerivativePow3, kernels.h:67
+        fx += dirx * b * snp / dj.y;    // cc:417
+        fy += diry * b * snp / dj.y;
+        fz += dirz * b * snp / dj.y;
+    });
+    float4 v = vs[k];
+    const float dt = S.dt, rho = di.x;
+    v.x = v.x + (fx / rho) * dt;  // cc:421
+    v.y = v.y + (fy / rho) * dt;
+    v.z = v.z + (fz / rho) * dt;
+    vs2[k] = v;
+}
+
+template <bool MULTI, bool WORK, bool LIST>
+__global__ __launch_bounds__(256) void k_viscosity_integrate(SPH_GATHER_PARAMS,
+                                                              const float4 *__restrict__ vs2,
+                                                              float4 *__restrict__ pos,
+                                                              float4 *__restrict__ vel)
+{
+    SPH_GATHER_PROLOGUE();
+    const float4 vi = vs2[k];
+    const float hh = S.hh, c6 = S.poly6c;
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    gather<LIST, true, true>(S, xps, vs2, start, endt, lst, k, pi,
+                             [&](uint32_t, float, float, float, float d2, const float4 &vj) {
+        const float d = sqrtf(d2);
+        const float v = hh - d * d;          // kernels.h:78
+        const float w = v * v * v * c6;      // kernels.h:79
+        fx += (vj.x - vi.x) * w;             // cc:460
+        fy += (vj.y - vi.y) * w;
+        fz += (vj.z - vi.z) * w;
+    });
+    const float dt = S.dt, mu = S.mu;
+    float4 v = vi;
+    v.x = vi.x + fx * mu * dt;  // cc:463
+    v.y = vi.y + fy * mu * dt;
+    v.z = vi.z + fz * mu * dt;
+    // stage F (cc:84-107), per particle
+    float4 x = pos[g];
+    x.x += v.x * dt;
+    x.y += v.y * dt;
+    x.z += v.z * dt;
+    if (S.half[0] - fabsf(x.x) <= 0.0f) { x.x = S.half[0] * sign_f(x.x); v.x *= S.damp_neg; }
+    if (S.half[1] - fabsf(x.y) <= 0.0f) { x.y = S.half[1] * sign_f(x.y); v.y *= S.damp_neg; }
+    if (S.half[2] - fabsf(x.z) <= 0.0f) { x.z = S.half[2] * sign_f(x.z); v.z *= S.damp_neg; }
+    x.w = 0.34f;  // OutPositions[i] = vec4(positions[i], 0.34f), cc:107
+    pos[g] = x;
+    vel[g] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -488,6 +644,12 @@ __global__ void k_neighbour_fill(const EngineDesc *__restrict__ ed,
     }
 }
 
+// Plain streaming copy: the reference point for HBM byte counters.
+__global__ void k_copy(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n)
+{
+    for (size_t i = global_id(); i < n; i += grid_stride()) dst[i] = src[i];
+}
+
 __global__ void k_count_nonfinite(const EngineDesc *__restrict__ ed,
                                   const float4 *__restrict__ pos, const float4 *__restrict__ vel,
                                   uint32_t *__restrict__ counter)
@@ -520,10 +682,10 @@ hipError_t launch_predict(const DevState &s, const LaunchCfg &c, hipStream_t st)
     const uint32_t g = grid_for(s.capacity, c.block_stream);
     if (s.multi)
         k_predict<true><<<g, c.block_stream, 0, st>>>(s.ed, s.sd, s.scene_of, s.pos, s.vel, s.xp,
-                                                       s.keys[0], s.vals[0]);
+                                                       s.keys[0], s.wkeys[0]);
     else
         k_predict<false><<<g, c.block_stream, 0, st>>>(s.ed, s.sd, s.scene_of, s.pos, s.vel, s.xp,
-                                                        s.keys[0], s.vals[0]);
+                                                        s.keys[0], s.wkeys[0]);
     SPH_LAUNCH_CHECK();
 }
 
@@ -534,16 +696,19 @@ hipError_t launch_clear_table(const DevState &s, const LaunchCfg &c, hipStream_t
     SPH_LAUNCH_CHECK();
 }
 
-hipError_t launch_radix_sort(const DevState &s, uint32_t bits, hipStream_t st, int *result_buf)
+hipError_t launch_radix_sort(const DevState &s, uint32_t *const keys[2], uint32_t *const vals[2],
+                             uint32_t bits, hipStream_t st, int *result_buf)
 {
     int cur = 0;
     const uint32_t nblk = s.radix_blocks;
     for (uint32_t shift = 0; shift < bits; shift += kRadixBits) {
-        k_radix_hist<<<nblk, kSortBlock, 0, st>>>(s.ed, s.keys[cur], shift, s.hist);
-        k_scan_exclusive<<<1, kScanBlock, 0, st>>>(s.hist, nblk * kRadix);
-        k_radix_scatter<<<nblk, kSortBlock, 0, st>>>(s.ed, s.keys[cur], s.vals[cur],
-                                                      s.keys[cur ^ 1], s.vals[cur ^ 1], shift,
-                                                      s.hist);
+        k_radix_hist<<<nblk, kSortBlock, 0, st>>>(s.ed, keys[cur], shift, s.hist);
+        k_radix_scan_digit<<<kRadix, kSortBlock, 0, st>>>(s.hist, nblk, s.hist + kRadix * nblk);
+        // first pass: values are the input slot indices, generated on the fly
+        k_radix_scatter<<<nblk, kSortBlock, 0, st>>>(s.ed, keys[cur], shift ? vals[cur] : nullptr,
+                                                      keys[cur ^ 1],
+                                                      vals[cur ^ 1], shift, s.hist,
+                                                      s.hist + kRadix * nblk);
         cur ^= 1;
     }
     *result_buf = cur;
@@ -553,44 +718,56 @@ hipError_t launch_radix_sort(const DevState &s, uint32_t bits, hipStream_t st, i
 hipError_t launch_reorder(const DevState &s, const LaunchCfg &c, int buf, hipStream_t st)
 {
     k_reorder<<<grid_for(s.capacity, c.block_stream), c.block_stream, 0, st>>>(
-        s.ed, s.keys[buf], s.vals[buf], s.xp, s.vel, s.xps, s.vs, s.start, s.end);
+        s.ed, s.keys[buf], s.vals[buf], s.xp, s.vel, s.xps, s.vs, s.start, s.end, s.rank);
     SPH_LAUNCH_CHECK();
 }
 
-hipError_t launch_density(const DevState &s, const LaunchCfg &c, hipStream_t st)
+// One work-item per slot; the grid is a multiple of 8 workgroups (XCD
+// renumbering of the WORK variants).
+inline uint32_t gather_grid(uint32_t cap, int block)
 {
-    const uint32_t g = grid_for(s.capacity, c.block_gather);
-    if (s.multi)
-        k_density<true><<<g, c.block_gather, 0, st>>>(s.ed, s.sd, s.scene_of, s.xps, s.start,
-                                                       s.end, s.dens);
-    else
-        k_density<false><<<g, c.block_gather, 0, st>>>(s.ed, s.sd, s.scene_of, s.xps, s.start,
-                                                        s.end, s.dens);
+    const uint32_t blocks = (cap + block - 1) / block;
+    return (blocks + 7u) & ~7u;
+}
+
+// Launches kernel template KER<MULTI, WORK, LIST> chosen at run time.
+#define SPH_GATHER_LAUNCH(KER, var, blk, st, ...)                                             \
+    do {                                                                                      \
+        const GatherVariant v_ = (var);                                                       \
+        const int b_ = (blk);                                                                 \
+        const uint32_t g_ = gather_grid(s.capacity, b_);                                      \
+        const size_t lds_ = v_.list ? sizeof(uint32_t) * kListMax * (size_t)b_ : 0;           \
+        const bool m_ = s.multi;                                                              \
+        if (m_ && v_.work && v_.list) KER<true, true, true><<<g_, b_, lds_, st>>>(__VA_ARGS__);      \
+        else if (m_ && v_.work) KER<true, true, false><<<g_, b_, lds_, st>>>(__VA_ARGS__);          \
+        else if (m_ && v_.list) KER<true, false, true><<<g_, b_, lds_, st>>>(__VA_ARGS__);          \
+        else if (m_) KER<true, false, false><<<g_, b_, lds_, st>>>(__VA_ARGS__);                    \
+        else if (v_.work && v_.list) KER<false, true, true><<<g_, b_, lds_, st>>>(__VA_ARGS__);     \
+        else if (v_.work) KER<false, true, false><<<g_, b_, lds_, st>>>(__VA_ARGS__);               \
+        else if (v_.list) KER<false, false, true><<<g_, b_, lds_, st>>>(__VA_ARGS__);               \
+        else KER<false, false, false><<<g_, b_, lds_, st>>>(__VA_ARGS__);                           \
+    } while (0)
+
+#define SPH_GATHER_ARGS s.ed, s.sd, s.scene_of, s.work, s.rank, s.vals[buf], s.xps, s.start, s.end
+
+hipError_t launch_density(const DevState &s, const LaunchCfg &c, int buf, hipStream_t st)
+{
+    SPH_GATHER_LAUNCH(k_density, c.density, c.density.block, st, SPH_GATHER_ARGS, s.dens);
     SPH_LAUNCH_CHECK();
 }
 
-hipError_t launch_pressure(const DevState &s, const LaunchCfg &c, hipStream_t st)
+hipError_t launch_pressure(const DevState &s, const LaunchCfg &c, int buf, hipStream_t st)
 {
-    const uint32_t g = grid_for(s.capacity, c.block_gather);
-    if (s.multi)
-        k_pressure<true><<<g, c.block_gather, 0, st>>>(s.ed, s.sd, s.scene_of, s.xps, s.start,
-                                                        s.end, s.dens, s.vs, s.vs2);
-    else
-        k_pressure<false><<<g, c.block_gather, 0, st>>>(s.ed, s.sd, s.scene_of, s.xps, s.start,
-                                                         s.end, s.dens, s.vs, s.vs2);
+    SPH_GATHER_LAUNCH(k_pressure, c.pressure, c.pressure.block, st, SPH_GATHER_ARGS, s.dens, s.vs,
+                      s.vs2);
     SPH_LAUNCH_CHECK();
 }
 
 hipError_t launch_viscosity_integrate(const DevState &s, const LaunchCfg &c, int buf,
                                       hipStream_t st)
 {
-    const uint32_t g = grid_for(s.capacity, c.block_gather);
-    if (s.multi)
-        k_viscosity_integrate<true><<<g, c.block_gather, 0, st>>>(
-            s.ed, s.sd, s.scene_of, s.xps, s.start, s.end, s.vs2, s.vals[buf], s.pos, s.vel);
-    else
-        k_viscosity_integrate<false><<<g, c.block_gather, 0, st>>>(
-            s.ed, s.sd, s.scene_of, s.xps, s.start, s.end, s.vs2, s.vals[buf], s.pos, s.vel);
+    SPH_GATHER_LAUNCH(k_viscosity_integrate, c.viscosity, c.viscosity.block, st, SPH_GATHER_ARGS,
+                      s.vs2, s.pos, s.vel);
     SPH_LAUNCH_CHECK();
 }
 
@@ -622,6 +799,12 @@ hipError_t launch_neighbour_fill(const DevState &s, int buf, const uint64_t *off
     else
         k_neighbour_fill<false><<<g, 256, 0, st>>>(s.ed, s.sd, s.scene_of, s.xps, s.start, s.end,
                                                    s.vals[buf], offs, list);
+    SPH_LAUNCH_CHECK();
+}
+
+hipError_t launch_copy(const float4 *src, float4 *dst, size_t n, hipStream_t st)
+{
+    k_copy<<<2048, 256, 0, st>>>(src, dst, n);
     SPH_LAUNCH_CHECK();
 }
 

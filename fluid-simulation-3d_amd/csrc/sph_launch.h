@@ -22,7 +22,7 @@ struct DevState {
     // sort
     uint32_t *keys[2];
     uint32_t *vals[2];
-    uint32_t *hist;        // [radix * radix_blocks]
+    uint32_t *hist;        // [radix * radix_blocks] counts, then [radix] digit totals
     uint32_t radix_blocks;
     // sorted order
     float4 *xps;   // predicted position, w = float(hash) (the reference record's .y)
@@ -31,13 +31,28 @@ struct DevState {
     float4 *dens;  // rho, rho_near, pressure, near pressure
     uint32_t *start;  // [table_total] first sorted slot of each bucket
     uint32_t *end;    // [table_total] one past the last slot (valid where start is)
+    // work order of the gather kernels
+    uint32_t *wkeys[2];  // Morton ordering keys (sort buffers)
+    uint32_t *wvals[2];  // particle ids (sort buffers); the sorted ones are the work order
+    const uint32_t *work;  // = wvals[result] of the last rebuild
+    uint32_t *rank;      // [capacity] sorted slot of each particle id
     uint32_t capacity;       // particle slots allocated
     uint32_t table_capacity; // bucket slots allocated
 };
 
+// Result-identical variants of a gather kernel (sph_kernels.hip explains them).
+struct GatherVariant {
+    bool work = false;  // Morton work order + XCD-contiguous workgroups
+    bool list = false;  // two phases through a per-lane LDS list
+    int block = 256;    // work-items per workgroup (64..256)
+};
+
 struct LaunchCfg {
-    int block_gather = 256;  // threads per block of density / pressure / viscosity
+    GatherVariant density{false, false, 256};
+    GatherVariant pressure{false, true, 128};
+    GatherVariant viscosity{false, false, 256};
     int block_stream = 256;  // threads per block of streaming kernels
+    bool needs_work() const { return density.work || pressure.work || viscosity.work; }
 };
 
 // Stage A + B1: predict, cell, hash, key; vals = slot index.
@@ -46,12 +61,13 @@ hipError_t launch_predict(const DevState &s, const LaunchCfg &c, hipStream_t st)
 hipError_t launch_clear_table(const DevState &s, const LaunchCfg &c, hipStream_t st);
 // Stage B2: stable LSD radix sort of (keys, vals) over `bits` key bits.
 // Returns the index (0/1) of the buffer pair holding the result.
-hipError_t launch_radix_sort(const DevState &s, uint32_t bits, hipStream_t st, int *result_buf);
+hipError_t launch_radix_sort(const DevState &s, uint32_t *const keys[2], uint32_t *const vals[2],
+                             uint32_t bits, hipStream_t st, int *result_buf);
 // Stage B3 + gather into sorted order.
 hipError_t launch_reorder(const DevState &s, const LaunchCfg &c, int buf, hipStream_t st);
 // Stage C, D, E+F.
-hipError_t launch_density(const DevState &s, const LaunchCfg &c, hipStream_t st);
-hipError_t launch_pressure(const DevState &s, const LaunchCfg &c, hipStream_t st);
+hipError_t launch_density(const DevState &s, const LaunchCfg &c, int buf, hipStream_t st);
+hipError_t launch_pressure(const DevState &s, const LaunchCfg &c, int buf, hipStream_t st);
 hipError_t launch_viscosity_integrate(const DevState &s, const LaunchCfg &c, int buf,
                                       hipStream_t st);
 
@@ -61,6 +77,8 @@ hipError_t launch_unpermute_dens(const DevState &s, int buf, float2 *out, hipStr
 hipError_t launch_neighbour_count(const DevState &s, int buf, uint32_t *counts, hipStream_t st);
 hipError_t launch_neighbour_fill(const DevState &s, int buf, const uint64_t *offs,
                                  uint32_t *list, hipStream_t st);
+// Plain copy of n float4 (HBM counter calibration).
+hipError_t launch_copy(const float4 *src, float4 *dst, size_t n, hipStream_t st);
 // Count of non-finite position/velocity components (error flag).
 hipError_t launch_count_nonfinite(const DevState &s, uint32_t *counter, hipStream_t st);
 

@@ -33,7 +33,8 @@ struct SceneDesc {
     float half[3];        // BoundScale * 0.5f (cc:88)
     float damp_neg;       // -1 * dampFactor (cc:94)
     float dt;             // substep dt of the current Update
-    uint32_t pad;
+    int32_t cell_lo[3];   // origin of the scene's cell grid for the work-order key
+    uint32_t pad[2];
 };
 
 // Whole-engine counters, resident in device memory so that a captured graph
@@ -43,4 +44,7 @@ struct EngineDesc {
     uint32_t table_total;  // buckets over all scenes
     uint32_t n_scenes;
     uint32_t key_bits;     // significant bits of the largest key (radix passes)
+    uint32_t scene_shift;  // work-order key: scene index << scene_shift (32: one scene)
+    uint32_t morton_drop;  // work-order key: Morton code >> morton_drop
+    uint32_t pad[2];
 };

@@ -106,11 +106,15 @@ def lib():
         "sph_stage_ms": (i32, [H, p]),
         "sph_count_nonfinite": (i32, [H, p]),
         "sph_profile_begin": (i32, [H, u32]),
+        "sph_autotune": (i32, [H, f, i32, p]),
+        "sph_get_variants": (i32, [H, p]),
+        "sph_set_variants": (i32, [H, p]),
         "sph_profile_end": (i32, [H, p, p]),
         "sph_debug_spatial": (i32, [H, p, p, p, p, p, p]),
         "sph_debug_predicted": (i32, [H, p]),
         "sph_debug_neighbours": (i32, [H, p, p, u64, p]),
         "sph_generate": (i32, [i32, u64, u32, p, f, f, f, p, p]),
+        "sph_copy_calibration": (i32, [u64, i32, p]),
         "sph_hash_cell": (u32, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
         "sph_bucket_of": (u32, [u32, u32]),
     }
@@ -157,6 +161,13 @@ def generate(kind, seed, n, box=(1.0, 1.0, 1.0), spacing=0.1, jitter=0.0, speed=
     b = np.asarray(box, np.float32)
     _check(lib().sph_generate(kind, seed, n, _ptr(b), spacing, jitter, speed, _ptr(pos), _ptr(vel)))
     return pos, vel
+
+
+def copy_calibration(nbytes, reps=3):
+    """Runs a plain device copy kernel of nbytes; returns ms per copy."""
+    ms = np.zeros(1, np.float64)
+    _check(lib().sph_copy_calibration(int(nbytes), int(reps), _ptr(ms)))
+    return float(ms[0])
 
 
 def hash_cell(cx, cy, cz):
@@ -253,6 +264,25 @@ class Sim:
         u = np.zeros(1, np.uint32)
         _check(lib().sph_profile_end(self._h, _ptr(ms), _ptr(u)))
         return ms, int(u[0])
+
+    def autotune(self, dt, updates=3):
+        ms = np.zeros(3, np.float64)
+        _check(lib().sph_autotune(self._h, dt, int(updates), _ptr(ms)))
+        return ms
+
+    @property
+    def variants(self):
+        v = np.zeros(9, np.int32)
+        _check(lib().sph_get_variants(self._h, _ptr(v)))
+        return {k: {"work": bool(v[3 * q]), "list": bool(v[3 * q + 1]), "block": int(v[3 * q + 2])}
+                for q, k in enumerate(("density", "pressure", "viscosity"))}
+
+    @variants.setter
+    def variants(self, d):
+        v = np.zeros(9, np.int32)
+        for q, k in enumerate(("density", "pressure", "viscosity")):
+            v[3 * q:3 * q + 3] = [int(d[k]["work"]), int(d[k]["list"]), int(d[k]["block"])]
+        _check(lib().sph_set_variants(self._h, _ptr(v)))
 
     def count_nonfinite(self):
         c = np.zeros(1, np.uint64)

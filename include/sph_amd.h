@@ -78,6 +78,11 @@ const char *sph_version(void);
 /* Number of visible HIP devices (0 without a GPU; never fails). */
 int sph_device_count(void);
 
+/* Measurement support, no reference counterpart: `reps` launches of a plain
+ * float4 copy kernel (k_copy) over `bytes` bytes; *ms_per_copy gets the mean
+ * duration.  Its byte counters calibrate per-dispatch FETCH_SIZE/WRITE_SIZE. */
+int sph_copy_calibration(uint64_t bytes, int reps, double *ms_per_copy);
+
 /* ---- single scene: FluidSimulation (physicsWorld.h:28-155) ---- */
 
 /* FluidSimulation::getInstance + InitializeData's sizing (h:32,36; cc:112-137):
@@ -125,6 +130,18 @@ int sph_stage_ms(sph_handle h, double ms[6]);
  * stage timers of physicsWorld.h:45-50. */
 int sph_profile_begin(sph_handle h, uint32_t max_updates);
 int sph_profile_end(sph_handle h, double mean_ms[6], uint32_t *updates);
+/* Autotuner (no reference counterpart): times every variant of the three
+ * gather kernels (work order on/off, LDS neighbour list on/off, workgroup
+ * size 64/128/256) on the current state with `updates` Update(dt) calls per
+ * candidate, restoring positions and velocities after each, and keeps the
+ * fastest variant per kernel.  All variants give bit-identical results.
+ * ms (may be NULL) gets the chosen variants' mean ms (density, pressure,
+ * viscosity).  Densities and debug tables are recomputed by the next update. */
+int sph_autotune(sph_handle h, float dt, int updates, double ms[3]);
+/* The variant of each gather kernel: v[3q+0] work order, v[3q+1] LDS list,
+ * v[3q+2] workgroup size, q = 0 density, 1 pressure, 2 viscosity. */
+int sph_get_variants(sph_handle h, int32_t v[9]);
+int sph_set_variants(sph_handle h, const int32_t v[9]);
 /* Number of non-finite position/velocity components in the current state. */
 int sph_count_nonfinite(sph_handle h, uint64_t *count);
 

@@ -32,7 +32,16 @@ def _cases():
         "uniform_64k_T2N": lambda: W.uniform_lower(n=65536, table=131072, box=(8.0, 2.0, 8.0)),
         "uniform_64k_T1000": lambda: W.uniform_lower(n=65536, table=1000, box=(8.0, 2.0, 8.0)),
         "uniform_64k_T2pow16_h035": lambda: _h035(),
+        "dense_lattice_overflow": lambda: _dense(),
     }
+
+
+def _dense():
+    """Lattice spacing 0.07 at h = 0.2: ~97 neighbours per particle, more than
+    the 64-entry neighbour list holds, so the walk fallback runs."""
+    n = 8192
+    pos, vel = S.generate(S.INIT_DAM_BREAK, 3, n, (4.0, 4.0, 4.0), 0.07, 0.001)
+    return W.Scene("dense_8192", W.scene_params(), n, n, pos, vel, 1)
 
 
 def _h035():
@@ -156,3 +165,46 @@ def test_initialize_data_matches_reference_lattice():
     op = orc_params(p)
     sp = O.spatial(pos_o, n, op.h)
     assert_bitexact(dens_g, O.density(op, pos_o, sp), "InitializeData densities")
+
+
+VARIANTS = [(w, l, b) for w in (0, 1) for l in (0, 1) for b in (64, 128, 256)]
+
+
+@pytest.mark.parametrize("case", ["dam_break_16k", "uniform_64k_T1000", "dense_lattice_overflow"])
+def test_every_gather_variant_bitexact(case):
+    """All work-order / LDS-list / workgroup-size variants give the oracle's bits."""
+    sc = _cases()[case]()
+    dt = _sub_dt()
+    ref = _oracle_update(sc, dt)
+    with S.Sim(sc.params, sc.n, sc.table) as sim:
+        for (w, l, b) in VARIANTS:
+            announce("variant", case, "work", w, "list", l, "block", b)
+            v = {"work": w, "list": l, "block": b}
+            sim.variants = {"density": v, "pressure": v, "viscosity": v}
+            sim.upload(sc.pos, sc.vel)
+            sim.update(dt)
+            pos_g, vel_g, dens_g = sim.download()
+            assert_bitexact(dens_g, ref["dens"], f"density {v}")
+            assert_bitexact(vel_g[:, :3], ref["vel"][:, :3], f"velocity {v}")
+            assert_bitexact(pos_g[:, :3], ref["pos"][:, :3], f"position {v}")
+
+
+def test_autotune_keeps_state_and_results():
+    sc = W.dam_break(n=16384)
+    dt = _sub_dt()
+    announce("autotune", sc.name)
+    with S.Sim(sc.params, sc.n, sc.table) as sim:
+        sim.upload(sc.pos, sc.vel)
+        ms = sim.autotune(dt, updates=2)
+        assert np.all(ms > 0)
+        pos_a, vel_a, _ = sim.download()
+        assert_bitexact(pos_a, sc.pos, "positions after autotune")
+        assert_bitexact(vel_a, sc.vel, "velocities after autotune")
+        for _ in range(2):
+            sim.step(W.DT, W.SUBSTEPS)
+        pos_g, vel_g, dens_g = sim.download()
+    pos_o, vel_o, dens_o = O.update(orc_params(sc.params), dt, sc.pos, sc.vel, T=sc.table,
+                                    substeps=2 * W.SUBSTEPS)
+    assert_bitexact(pos_g[:, :3], pos_o[:, :3], "positions after autotune + 2 steps")
+    assert_bitexact(vel_g[:, :3], vel_o[:, :3], "velocities after autotune + 2 steps")
+    assert_bitexact(dens_g, dens_o, "densities after autotune + 2 steps")
