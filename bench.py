@@ -66,6 +66,120 @@ def copy_calibration(mib):
         f"({2 * (mib << 20) / (ms * 1e-3) / 1e9:.0f} GB/s read+write)")
 
 
+def bench_batch(steps, plain_steps, warmup):
+    """Batched (one launch per kernel for all scenes, graph-replayed) vs plain
+    (one handle per scene, each kernel launch covering one scene, all handles
+    enqueued asynchronously) on the same 256-scene workload."""
+    bs = W.batch()
+    total = bs.total
+    log(f"[bench] batch: {len(bs.scenes)} scenes, {total} particles")
+    b = S.Batch([sc.params for sc in bs.scenes], [sc.n for sc in bs.scenes],
+                [sc.table for sc in bs.scenes])
+    for s_, sc in enumerate(bs.scenes):
+        b.upload(s_, sc.pos, sc.vel)
+    for _ in range(warmup):
+        b.step_async(bs.dt, bs.substeps)
+    b.sync()
+    sub_dt = float(np.float32(np.float32(bs.dt) / np.float32(bs.substeps)))
+    b.autotune(sub_dt, updates=3)
+    b.step(bs.dt, bs.substeps)
+    b.profile_begin(steps * bs.substeps)
+    b.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.step_async(bs.dt, bs.substeps)
+    b.sync()
+    el_b = time.perf_counter() - t0
+    stage_ms, _ = b.profile_end()
+    lat = []
+    for _ in range(20):
+        t = time.perf_counter()
+        b.step(bs.dt, bs.substeps)
+        lat.append((time.perf_counter() - t) * 1e3)
+    variants = b.variants
+    b.close()
+    log(f"[bench] batched {steps} steps in {el_b:.3f} s")
+
+    sims = []
+    for sc in bs.scenes:
+        sim = S.Sim(sc.params, sc.n, sc.table)
+        sim.upload(sc.pos, sc.vel)
+        sims.append(sim)
+    for sim in sims:
+        sim.step_async(bs.dt, bs.substeps)
+    for sim in sims:
+        sim.sync()
+    t0 = time.perf_counter()
+    for _ in range(plain_steps):
+        for sim in sims:
+            sim.step_async(bs.dt, bs.substeps)
+        for sim in sims:
+            sim.sync()
+    el_p = time.perf_counter() - t0
+    for sim in sims:
+        sim.close()
+    log(f"[bench] plain {plain_steps} steps in {el_p:.3f} s")
+    v_b = total * steps / el_b
+    v_p = total * plain_steps / el_p
+    return {"workload": bs.name, "scenes": len(bs.scenes), "particles": total,
+            "value": v_b, "unit": "particle-steps/s", "ms_per_step": el_b * 1e3 / steps,
+            "steps": steps,
+            "plain_one_scene_per_launch": {"value": v_p, "ms_per_step": el_p * 1e3 / plain_steps,
+                                           "steps": plain_steps},
+            "speedup_vs_plain": v_b / v_p,
+            "step_latency_ms": {"p50": float(np.percentile(lat, 50)),
+                                "p99": float(np.percentile(lat, 99)), "n": len(lat)},
+            "stage_ms_per_update": {STAGES[i]: float(stage_ms[i]) for i in range(5)},
+            "gather_variants": variants}
+
+
+def bench_serve(duration, rate, max_running, budget_gib):
+    """Poisson arrivals of bake requests for `duration` seconds of wall time,
+    served by continuous batching (re-batched every substep).  Requests are
+    generated at their arrival time; the timed window is the whole duration."""
+    arrivals = W.serving_requests(duration, rate=rate)
+    srv = S.Server(hbm_budget_bytes=budget_gib << 30, max_running=max_running,
+                   initial_capacity=1 << 25, updates_per_tick=1)
+    log(f"[bench] serve: {len(arrivals)} arrivals over {duration} s, slots {max_running}")
+    nxt, ticks, submitted = 0, [], 0
+    offered = sum(n * st for (_, n, st, _, _) in arrivals) / duration
+    t0 = time.perf_counter()
+    while True:
+        now = time.perf_counter() - t0
+        if now >= duration:
+            break
+        while nxt < len(arrivals) and arrivals[nxt][0] <= now:
+            _, n, steps, kind, seed = arrivals[nxt]
+            p, pos, vel = W.serving_scene(n, kind, seed)
+            srv.submit(p, pos, vel, steps)
+            submitted += 1
+            nxt += 1
+        info = srv.tick()
+        if info["particles"]:
+            ticks.append(info["ms"])
+        elif nxt < len(arrivals):
+            time.sleep(max(0.0, min(0.01, arrivals[nxt][0] - (time.perf_counter() - t0))))
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    tk = np.array(ticks)
+    # one step = `substeps` consecutive ticks (each tick is one Update of every request)
+    step_ms = np.convolve(tk, np.ones(3), "valid") if tk.size >= 3 else tk
+    psteps = st["particle_updates"] / 3.0
+    return {"workload": "serving", "arrivals_per_s": rate, "submitted": submitted,
+            "admitted": st["admitted"], "finished": st["finished"], "queued_at_end": st["queued"],
+            "running_at_end": st["running"], "slots": max_running, "hbm_budget_gib": budget_gib,
+            "value": psteps / wall, "unit": "particle-steps/s", "wall_s": wall,
+            "offered_particle_steps_per_s": offered,
+            "step_latency_ms": {"p50": float(np.percentile(step_ms, 50)) if step_ms.size else None,
+                                "p99": float(np.percentile(step_ms, 99)) if step_ms.size else None},
+            "substep_tick_ms": {"p50": st["tick_ms_p50"], "p99": st["tick_ms_p99"]},
+            "request_latency_ms": {"p50": st["request_ms_p50"], "p99": st["request_ms_p99"],
+                                   "n": st["finished"]},
+            "queue_wait_ms": {"p50": st["wait_ms_p50"], "p99": st["wait_ms_p99"]},
+            "device_capacity_particles": st["capacity"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +190,13 @@ def main():
     ap.add_argument("--cpu-substeps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=30)
+    ap.add_argument("--plain-steps", type=int, default=5)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=15.0)
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-slots", type=int, default=64)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -157,6 +278,10 @@ def main():
                      "alg_bytes_per_particle": ALG_BYTES[dom]},
         "nonfinite_components": nonfinite,
     }
+    if not args.no_batch:
+        out["batch"] = bench_batch(args.batch_steps, args.plain_steps, 3)
+    if not args.no_serve:
+        out["serve"] = bench_serve(args.serve_seconds, args.serve_rate, args.serve_slots, 200)
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_substeps)

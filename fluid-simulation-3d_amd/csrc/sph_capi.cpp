@@ -14,8 +14,23 @@ struct sph_sim {
     sph::Engine *eng;
 };
 
+struct sph_batch_t {
+    sph::Engine *eng;
+};
+
 namespace {
 thread_local std::string g_err;
+}  // namespace
+
+namespace sph {
+int set_last_error(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+}  // namespace sph
+
+namespace {
 
 int fail(int code, const std::string &msg)
 {
@@ -285,36 +300,151 @@ int sph_autotune(sph_handle h, float dt, int updates, double ms[3])
     SPH_GUARD_END
 }
 
-int sph_get_variants(sph_handle h, int32_t v[9])
+namespace {
+int get_variants(sph::Engine *e, int32_t v[13])
 {
-    SPH_NEED(h);
     if (!v) return fail(SPH_ERR_INVALID, "null output");
-    const sph::LaunchCfg &c = h->eng->launch_cfg();
+    const sph::LaunchCfg &c = e->launch_cfg();
     const sph::GatherVariant g[3] = {c.density, c.pressure, c.viscosity};
     for (int q = 0; q < 3; ++q) {
-        v[3 * q + 0] = g[q].work;
-        v[3 * q + 1] = g[q].list;
-        v[3 * q + 2] = g[q].block;
+        v[4 * q + 0] = g[q].work;
+        v[4 * q + 1] = g[q].list;
+        v[4 * q + 2] = g[q].block;
+        v[4 * q + 3] = g[q].rows;
     }
+    v[12] = c.nbr;
     return SPH_OK;
 }
 
-int sph_set_variants(sph_handle h, const int32_t v[9])
+int set_variants(sph::Engine *e, const int32_t v[13])
 {
-    SPH_NEED(h);
     if (!v) return fail(SPH_ERR_INVALID, "null input");
-    sph::LaunchCfg c = h->eng->launch_cfg();
+    sph::LaunchCfg c = e->launch_cfg();
+    c.nbr = v[12] != 0;
     sph::GatherVariant *g[3] = {&c.density, &c.pressure, &c.viscosity};
     for (int q = 0; q < 3; ++q) {
-        const int b = v[3 * q + 2];
+        const int b = v[4 * q + 2];
         if (b != 64 && b != 128 && b != 192 && b != 256)
             return fail(SPH_ERR_INVALID, "block must be 64, 128, 192 or 256");
-        g[q]->work = v[3 * q + 0] != 0;
-        g[q]->list = v[3 * q + 1] != 0;
+        g[q]->work = v[4 * q + 0] != 0;
+        g[q]->list = v[4 * q + 1] != 0;
         g[q]->block = b;
+        g[q]->rows = v[4 * q + 3] != 0;
     }
-    h->eng->set_launch_cfg(c);
+    if ((c.pressure.rows || c.viscosity.rows) && !c.nbr)
+        return fail(SPH_ERR_INVALID, "reading neighbour rows needs the row mode (v[12])");
+    e->set_launch_cfg(c);
     return SPH_OK;
+}
+}  // namespace
+
+int sph_get_variants(sph_handle h, int32_t v[13])
+{
+    SPH_NEED(h);
+    return get_variants(h->eng, v);
+}
+
+int sph_set_variants(sph_handle h, const int32_t v[13])
+{
+    SPH_NEED(h);
+    return set_variants(h->eng, v);
+}
+
+// ---- batches ----
+
+int sph_batch_create(uint32_t n_scenes, const uint32_t *counts, const uint32_t *table_sizes,
+                     const sph_params *params, sph_batch *out)
+{
+    SPH_GUARD_BEGIN
+    if (!out || !counts || !params || n_scenes == 0)
+        return fail(SPH_ERR_INVALID, "need n_scenes > 0, counts, params and out");
+    *out = nullptr;
+    if (!have_gpu()) return fail(SPH_ERR_HIP, "no HIP device visible");
+    std::vector<sph::SceneSpec> specs(n_scenes);
+    for (uint32_t s = 0; s < n_scenes; ++s) {
+        const uint32_t t = table_sizes && table_sizes[s] ? table_sizes[s] : counts[s];
+        specs[s] = sph::SceneSpec{params[s], counts[s], t};
+    }
+    std::string err;
+    sph::Engine *e = nullptr;
+    int rc = sph::Engine::create(specs, 0, 0, n_scenes, &e, &err);
+    if (rc) return fail(rc, err);
+    *out = new sph_batch_t{e};
+    return SPH_OK;
+    SPH_GUARD_END
+}
+
+int sph_batch_destroy(sph_batch b)
+{
+    SPH_GUARD_BEGIN
+    if (!b) return SPH_OK;
+    delete b->eng;
+    delete b;
+    return SPH_OK;
+    SPH_GUARD_END
+}
+
+#define SPH_BATCH_CALL(expr)                     \
+    SPH_GUARD_BEGIN                              \
+    SPH_NEED(b);                                 \
+    std::string err;                             \
+    return done((expr), err);                    \
+    SPH_GUARD_END
+
+int sph_batch_upload(sph_batch b, uint32_t scene, const float *pos4, const float *vel4)
+{
+    SPH_BATCH_CALL(b->eng->upload(scene, pos4, vel4, &err))
+}
+
+int sph_batch_set_params(sph_batch b, uint32_t scene, const sph_params *p)
+{
+    if (!p) return fail(SPH_ERR_INVALID, "null params");
+    SPH_BATCH_CALL(b->eng->set_params(scene, *p, &err))
+}
+
+int sph_batch_step(sph_batch b, float dt, int substeps)
+{
+    SPH_BATCH_CALL(b->eng->step(dt, substeps, true, &err))
+}
+
+int sph_batch_step_async(sph_batch b, float dt, int substeps)
+{
+    SPH_BATCH_CALL(b->eng->step(dt, substeps, false, &err))
+}
+
+int sph_batch_sync(sph_batch b) { SPH_BATCH_CALL(b->eng->sync(&err)) }
+
+int sph_batch_download(sph_batch b, uint32_t scene, float *pos4, float *vel4, float *dens2)
+{
+    SPH_BATCH_CALL(b->eng->download(scene, pos4, vel4, dens2, &err))
+}
+
+int sph_batch_autotune(sph_batch b, float dt, int updates, double ms[3])
+{
+    SPH_BATCH_CALL(b->eng->autotune(dt, updates, ms, &err))
+}
+
+int sph_batch_profile_begin(sph_batch b, uint32_t max_updates)
+{
+    SPH_BATCH_CALL(b->eng->profile_begin(max_updates, &err))
+}
+
+int sph_batch_profile_end(sph_batch b, double mean_ms[6], uint32_t *updates)
+{
+    if (!mean_ms) return fail(SPH_ERR_INVALID, "null output");
+    SPH_BATCH_CALL(b->eng->profile_end(mean_ms, updates, &err))
+}
+
+int sph_batch_get_variants(sph_batch b, int32_t v[13])
+{
+    SPH_NEED(b);
+    return get_variants(b->eng, v);
+}
+
+int sph_batch_set_variants(sph_batch b, const int32_t v[13])
+{
+    SPH_NEED(b);
+    return set_variants(b->eng, v);
 }
 
 int sph_count_nonfinite(sph_handle h, uint64_t *count)

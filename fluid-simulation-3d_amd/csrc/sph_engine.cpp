@@ -113,10 +113,10 @@ int Engine::check(hipError_t e, const char *what, std::string *err)
 
 int Engine::create(const std::vector<SceneSpec> &scenes, uint32_t capacity,
                    uint32_t table_capacity, uint32_t scene_capacity, Engine **out,
-                   std::string *err)
+                   std::string *err, bool multi)
 {
-    if (scenes.empty()) {
-        *err = "no scenes";
+    if (scenes.empty() && (capacity == 0 || table_capacity == 0 || scene_capacity == 0)) {
+        *err = "no scenes and no capacities";
         return SPH_ERR_INVALID;
     }
     uint64_t n_total = 0, t_total = 0;
@@ -168,7 +168,7 @@ int Engine::create(const std::vector<SceneSpec> &scenes, uint32_t capacity,
     e->key_bits_ = bits_for(table_capacity - 1);
     e->ed_h_->key_bits = e->key_bits_;
     work_key_layout(e->ed_h_->n_scenes, e->ed_h_);
-    e->d_.multi = scenes.size() > 1;
+    e->d_.multi = multi || scenes.size() > 1;
     rc = e->check(hipMemcpy(e->d_scene_of_, scene_of.data(), sizeof(uint32_t) * capacity,
                             hipMemcpyHostToDevice),
                   "upload scene map", err);
@@ -238,6 +238,8 @@ int Engine::alloc(uint32_t cap, uint32_t tcap, std::string *err)
         if (!rc) rc = dm((void **)&d_.wvals[b], u4, "work order");
     }
     if (!rc) rc = dm((void **)&d_.rank, u4, "rank");
+    if (!rc) rc = dm((void **)&d_.nbr, u4 * 64, "neighbour rows");
+    if (!rc) rc = dm((void **)&d_.ncnt, u4, "neighbour counts");
     d_.ed = d_ed_;
     d_.sd = d_sd_;
     d_.scene_of = d_scene_of_;
@@ -252,9 +254,12 @@ Engine::~Engine()
     void *bufs[] = {d_ed_,       d_sd_,       d_scene_of_, d_.pos,     d_.vel,    d_.xp,
                     d_.xps,      d_.vs,       d_.vs2,      d_.dens,    d_.keys[0], d_.keys[1],
                     d_.vals[0],  d_.vals[1],  d_.hist,     d_.start,   d_.end,
-                    d_.wkeys[0], d_.wkeys[1], d_.wvals[0], d_.wvals[1], d_.rank};
+                    d_.wkeys[0], d_.wkeys[1], d_.wvals[0], d_.wvals[1], d_.rank,
+                    d_.nbr,      d_.ncnt};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
+    if (tmp_pos_) (void)hipFree(tmp_pos_);
+    if (tmp_vel_) (void)hipFree(tmp_vel_);
     if (ed_h_) (void)hipHostFree(ed_h_);
     if (sd_pinned_) (void)hipHostFree(sd_pinned_);
     for (hipEvent_t &e : ev_)
@@ -287,6 +292,97 @@ int Engine::push_descs(std::string *err)
            "upload scene table");
     CK(hipMemcpyAsync(d_ed_, ed_h_, sizeof(EngineDesc), hipMemcpyHostToDevice, stream_),
        "upload engine desc");
+    return SPH_OK;
+}
+
+int Engine::repack(const std::vector<RepackItem> &items, std::string *err)
+{
+    uint64_t n_total = 0, t_total = 0;
+    for (const RepackItem &it : items) {
+        if (it.spec.n == 0 || it.spec.table == 0) {
+            *err = "scene needs n > 0 and table_size > 0";
+            return SPH_ERR_INVALID;
+        }
+        if (const char *m = check_params(it.spec.params)) {
+            *err = m;
+            return SPH_ERR_INVALID;
+        }
+        if (it.old_scene >= (int)sd_h_.size() ||
+            (it.old_scene >= 0 && sd_h_[it.old_scene].n != it.spec.n) ||
+            (it.old_scene < 0 && (!it.pos4 || !it.vel4))) {
+            *err = "repack item does not match an existing scene or lacks its state";
+            return SPH_ERR_INVALID;
+        }
+        n_total += it.spec.n;
+        t_total += it.spec.table;
+    }
+    if (n_total > d_.capacity || t_total > d_.table_capacity || items.size() > scene_cap_) {
+        *err = "repack exceeds the engine's capacity";
+        return SPH_ERR_INVALID;
+    }
+    const size_t cap_bytes = sizeof(float4) * (size_t)d_.capacity;
+    if (!tmp_pos_) CK(hipMalloc((void **)&tmp_pos_, cap_bytes), "repack staging");
+    if (!tmp_vel_) CK(hipMalloc((void **)&tmp_vel_, cap_bytes), "repack staging");
+    device_bytes_ += 0;
+    // 1. moved scenes: old place -> staging; 2. staging -> new place; new
+    // scenes: host -> new place.  Scenes that keep their offset do not move.
+    std::vector<uint32_t> new_off(items.size());
+    uint32_t off = 0;
+    for (size_t q = 0; q < items.size(); ++q) {
+        new_off[q] = off;
+        off += items[q].spec.n;
+    }
+    const size_t f4 = sizeof(float4);
+    for (size_t q = 0; q < items.size(); ++q) {
+        const RepackItem &it = items[q];
+        if (it.old_scene < 0 || sd_h_[it.old_scene].p_off == new_off[q]) continue;
+        const uint32_t o = sd_h_[it.old_scene].p_off, n = it.spec.n;
+        CK(hipMemcpyAsync(tmp_pos_ + new_off[q], d_.pos + o, f4 * n, hipMemcpyDeviceToDevice, stream_),
+           "repack move");
+        CK(hipMemcpyAsync(tmp_vel_ + new_off[q], d_.vel + o, f4 * n, hipMemcpyDeviceToDevice, stream_),
+           "repack move");
+    }
+    for (size_t q = 0; q < items.size(); ++q) {
+        const RepackItem &it = items[q];
+        const uint32_t n = it.spec.n, o = new_off[q];
+        if (it.old_scene < 0) {
+            CK(hipMemcpyAsync(d_.pos + o, it.pos4, f4 * n, hipMemcpyHostToDevice, stream_), "repack upload");
+            CK(hipMemcpyAsync(d_.vel + o, it.vel4, f4 * n, hipMemcpyHostToDevice, stream_), "repack upload");
+        } else if (sd_h_[it.old_scene].p_off != o) {
+            CK(hipMemcpyAsync(d_.pos + o, tmp_pos_ + o, f4 * n, hipMemcpyDeviceToDevice, stream_), "repack move");
+            CK(hipMemcpyAsync(d_.vel + o, tmp_vel_ + o, f4 * n, hipMemcpyDeviceToDevice, stream_), "repack move");
+        }
+    }
+    // descriptors of the new layout
+    std::vector<SceneDesc> sd(items.size());
+    std::vector<sph_params> params(items.size());
+    std::vector<uint32_t> scene_of(n_total);
+    uint32_t k_off = 0;
+    for (size_t q = 0; q < items.size(); ++q) {
+        const RepackItem &it = items[q];
+        fill_scene_desc(it.spec.params, it.spec.n, it.spec.table, &sd[q]);
+        sd[q].p_off = new_off[q];
+        sd[q].key_off = k_off;
+        sd[q].dt = dt_cached_ < 0.0f ? 0.0f : dt_cached_;
+        k_off += it.spec.table;
+        params[q] = it.spec.params;
+        for (uint32_t i = 0; i < it.spec.n; ++i) scene_of[new_off[q] + i] = (uint32_t)q;
+    }
+    if (n_total)
+        CK(hipMemcpyAsync(d_scene_of_, scene_of.data(), sizeof(uint32_t) * n_total,
+                          hipMemcpyHostToDevice, stream_), "repack scene map");
+    // host sources (caller buffers, scene_of) must stay alive until the copies end
+    CK(hipStreamSynchronize(stream_), "repack sync");
+    sd_h_.swap(sd);
+    params_.swap(params);
+    ed_h_->n_total = (uint32_t)n_total;
+    ed_h_->table_total = (uint32_t)t_total;
+    ed_h_->n_scenes = (uint32_t)items.size();
+    work_key_layout(ed_h_->n_scenes, ed_h_);
+    int rc = push_descs(err);
+    if (rc) return rc;
+    work_valid_ = false;
+    have_tables_ = false;
     return SPH_OK;
 }
 
@@ -436,16 +532,27 @@ int Engine::autotune(float dt, int updates, double ms_out[3], std::string *err)
     const LaunchCfg keep = cfg_;
     const bool keep_timing = timing_;
     timing_ = false;
-    std::vector<GatherVariant> cands;
+    // Candidates: every (work, list, block) for the walk form of all three
+    // kernels, then the neighbour-row mode (density writes rows, pressure and
+    // viscosity read them) for each block size.  Per-kernel times combine:
+    // walk mode = best walk variant of each kernel; row mode = best row-writing
+    // density + for pressure and viscosity the better of rows and walk.
+    struct Cand {
+        GatherVariant v;
+        bool nbr;
+    };
+    std::vector<Cand> cands;
     for (int w = 0; w < 2; ++w)
         for (int l = 0; l < 2; ++l)
-            for (int b : {64, 128, 256}) cands.push_back(GatherVariant{w != 0, l != 0, b});
-    double best[3] = {1e30, 1e30, 1e30};
-    GatherVariant pick[3] = {keep.density, keep.pressure, keep.viscosity};
+            for (int b : {64, 128, 256}) cands.push_back({GatherVariant{w != 0, l != 0, b, false}, false});
+    for (int b : {64, 128, 256}) cands.push_back({GatherVariant{true, false, b, true}, true});
+    double walk_ms[3] = {1e30, 1e30, 1e30}, row_ms[3] = {1e30, 1e30, 1e30};
+    GatherVariant walk_v[3] = {keep.density, keep.pressure, keep.viscosity}, row_v[3];
     int rc = check(save(), "autotune snapshot", err);
     for (size_t ci = 0; ci < cands.size() && !rc; ++ci) {
         LaunchCfg c = keep;
-        c.density = c.pressure = c.viscosity = cands[ci];
+        c.density = c.pressure = c.viscosity = cands[ci].v;
+        c.nbr = cands[ci].nbr;
         cfg_ = c;
         invalidate_graph();
         work_valid_ = false;
@@ -457,11 +564,35 @@ int Engine::autotune(float dt, int updates, double ms_out[3], std::string *err)
         uint32_t used = 0;
         if (!rc) rc = profile_end(ms, &used, err);
         if (!rc) rc = check(restore(), "autotune restore", err);
-        for (int q = 0; q < 3 && !rc; ++q)
+        if (rc) break;
+        double *best = cands[ci].nbr ? row_ms : walk_ms;
+        GatherVariant *pick = cands[ci].nbr ? row_v : walk_v;
+        for (int q = 0; q < 3; ++q)
             if (ms[2 + q] < best[q]) {
                 best[q] = ms[2 + q];
-                pick[q] = cands[ci];
+                pick[q] = cands[ci].v;
             }
+    }
+    const double walk_total = walk_ms[0] + walk_ms[1] + walk_ms[2];
+    GatherVariant rp = row_ms[1] < walk_ms[1] ? row_v[1] : walk_v[1];
+    GatherVariant rv = row_ms[2] < walk_ms[2] ? row_v[2] : walk_v[2];
+    const double row_total = row_ms[0] + std::min(row_ms[1], walk_ms[1]) + std::min(row_ms[2], walk_ms[2]);
+    const bool use_nbr = (rp.rows || rv.rows) && row_total < walk_total;
+    double best[3];
+    GatherVariant pick[3];
+    if (use_nbr) {
+        pick[0] = row_v[0];
+        pick[1] = rp;
+        pick[2] = rv;
+        best[0] = row_ms[0];
+        best[1] = std::min(row_ms[1], walk_ms[1]);
+        best[2] = std::min(row_ms[2], walk_ms[2]);
+    } else {
+        for (int q = 0; q < 3; ++q) {
+            pick[q] = walk_v[q];
+            pick[q].rows = false;
+            best[q] = walk_ms[q];
+        }
     }
     if (!rc) rc = check(hipStreamSynchronize(stream_), "autotune sync", err);
     (void)hipFree(snap);
@@ -471,6 +602,7 @@ int Engine::autotune(float dt, int updates, double ms_out[3], std::string *err)
         chosen.density = pick[0];
         chosen.pressure = pick[1];
         chosen.viscosity = pick[2];
+        chosen.nbr = use_nbr;
     }
     cfg_ = chosen;
     invalidate_graph();

@@ -19,6 +19,15 @@ struct SceneSpec {
     uint32_t table;
 };
 
+// One scene of a repacked layout: an existing scene (old_scene >= 0, its
+// state moves on the device) or a new one (state uploaded from host).
+struct RepackItem {
+    SceneSpec spec;
+    int old_scene;
+    const float *pos4;
+    const float *vel4;
+};
+
 // Fills the derived constants of a SceneDesc from user settings, with the
 // reference's float expressions (kernels.h:25-82, physicsWorld.cc:47,87-88).
 void fill_scene_desc(const sph_params &p, uint32_t n, uint32_t table, SceneDesc *d);
@@ -30,9 +39,19 @@ const char *check_params(const sph_params &p);
 class Engine {
   public:
     // capacity: particle slots to allocate (>= total of scenes); table_capacity likewise.
+    // multi: always run the multi-scene kernels (a layout that changes over
+    // time may hold one scene now and many later).  An empty `scenes` is
+    // allowed with explicit capacities (a server starts empty).
     static int create(const std::vector<SceneSpec> &scenes, uint32_t capacity,
                       uint32_t table_capacity, uint32_t scene_capacity, Engine **out,
-                      std::string *err);
+                      std::string *err, bool multi = false);
+    // Replaces the scene list in place: buffers, grids and captured graphs stay
+    // valid (the kernels read the counts from device memory).  Fails without
+    // changes if the items exceed the capacities.
+    int repack(const std::vector<RepackItem> &items, std::string *err);
+    uint32_t capacity() const { return d_.capacity; }
+    uint32_t table_capacity() const { return d_.table_capacity; }
+    uint32_t scene_capacity() const { return scene_cap_; }
     ~Engine();
 
     uint32_t n_scenes() const { return (uint32_t)sd_h_.size(); }
@@ -124,6 +143,8 @@ class Engine {
     double stage_ms_[6] = {0, 0, 0, 0, 0, 0};
     uint64_t device_bytes_ = 0;
     float dt_cached_ = -1.0f;
+    float4 *tmp_pos_ = nullptr;  // repack staging
+    float4 *tmp_vel_ = nullptr;
     std::vector<hipEvent_t> prof_ev_;
     uint32_t prof_used_ = 0;
     bool prof_active_ = false;

@@ -467,26 +467,127 @@ __device__ __forceinline__ void gather(const SceneDesc &S, const float4 *__restr
         const uint32_t *__restrict__ scene_of, const uint32_t *__restrict__ work,             \
         const uint32_t *__restrict__ rank, const uint32_t *__restrict__ svals,                \
         const float4 *__restrict__ xps, const uint32_t *__restrict__ start,                   \
-        const uint32_t *__restrict__ endt
+        const uint32_t *__restrict__ endt, uint32_t *__restrict__ nbr,                        \
+        uint32_t *__restrict__ ncnt
 
-template <bool MULTI, bool WORK, bool LIST>
+// NBR (with WORK): the density pass also stores each particle's neighbours
+// with d < h (self excluded, walk order) as row i = work index of the
+// particle: nbr[i * kNbrMax + j], count in ncnt[i].  A wave's rows form one
+// contiguous block, so the writes stay in L2 until whole.  Pressure and
+// viscosity then loop over the row instead of walking the buckets; their
+// neighbour reads are spatially local in the work order.  A particle with
+// more than kNbrMax neighbours walks the buckets instead.
+constexpr uint32_t kNbrMax = 64;
+
+template <typename Body>
+__device__ __forceinline__ void gather_row(const SceneDesc &S, const float4 *__restrict__ xps,
+                                           const float4 *__restrict__ aux,
+                                           const uint32_t *__restrict__ start,
+                                           const uint32_t *__restrict__ endt,
+                                           const uint32_t *__restrict__ row, uint32_t c,
+                                           uint32_t k, const float4 &pi, Body &&body)
+{
+    if (c > kNbrMax) {
+        walk<true, true>(S, xps, aux, start, endt, k, pi, body);
+        return;
+    }
+    const uint4 *row4 = reinterpret_cast<const uint4 *>(row);
+    for (uint32_t j = 0; j < c; j += 4) {
+        const uint4 mm = row4[j >> 2];
+        const uint32_t m[4] = {mm.x, mm.y, mm.z, mm.w};
+        float4 r[4], a[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            r[q] = pi;
+            a[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (j + q < c) {
+                r[q] = xps[m[q]];
+                a[q] = aux[m[q]];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (j + q < c) {
+                const float ox = r[q].x - pi.x, oy = r[q].y - pi.y, oz = r[q].z - pi.z;
+                body(m[q], ox, oy, oz, ox * ox + oy * oy + oz * oz, a[q]);
+            }
+    }
+}
+
+template <bool MULTI, bool WORK, bool LIST, bool NBR>
 __global__ __launch_bounds__(256) void k_density(SPH_GATHER_PARAMS, float4 *__restrict__ dens)
 {
     SPH_GATHER_PROLOGUE();
     const float h = S.h, w2c = S.w2c, w3c = S.w3c;
     float rho = 0.0f, rhon = 0.0f;
+    uint32_t *row = NBR ? nbr + (size_t)i * kNbrMax : nullptr;
+    uint32_t c = 0;
     gather<LIST, false, false>(S, xps, nullptr, start, endt, lst, k, pi,
-                               [&](uint32_t, float, float, float, float d2, const float4 &) {
+                               [&](uint32_t m, float, float, float, float d2, const float4 &) {
         const float d = sqrtf(d2);
         const float v = h - d;
         rho += v * v * w2c;       // SmoothingPow2, kernels.h:31
         rhon += v * v * v * w3c;  // SmoothingPow3, kernels.h:43
+        if (NBR && m != k) {
+            if (c < kNbrMax) row[c] = m;
+            ++c;
+        }
     });
+    if (NBR) ncnt[i] = c;
     // pressures of cc:371-372 / 407-408, computed once per particle
     dens[k] = make_float4(rho, rhon, (rho - S.rho0) * S.k, rhon * S.k_near);
 }
 
-template <bool MULTI, bool WORK, bool LIST>
+// Density pass of the neighbour-row mode.  Each work-item stages its row in
+// LDS (row-major with a 65-word pitch, so the 64 lanes writing entry c of
+// their rows hit 32 distinct banks); after a barrier the workgroup writes its
+// rows -- one contiguous block of global memory -- with coalesced 16-byte
+// stores, skipping the chunks beyond each row's count.
+constexpr uint32_t kRowPitch = kNbrMax + 1;
+
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_density_rows(SPH_GATHER_PARAMS, float4 *__restrict__ dens)
+{
+    extern __shared__ uint32_t lds_rows[];  // [blockDim.x][kRowPitch], then counts[blockDim.x]
+    const uint32_t B = blockDim.x, t = threadIdx.x;
+    const uint32_t i = gather_index<true>();
+    const uint32_t n = ed->n_total;
+    uint32_t *my = lds_rows + t * kRowPitch;
+    uint32_t *cnt = lds_rows + B * kRowPitch;
+    uint32_t c = 0;
+    if (i < n) {
+        const uint32_t k = rank[work[i]];
+        const SceneDesc &S = scene_of_slot<MULTI>(sd, scene_of, k);
+        const float4 pi = xps[k];
+        const float h = S.h, w2c = S.w2c, w3c = S.w3c;
+        float rho = 0.0f, rhon = 0.0f;
+        walk<false, false>(S, xps, nullptr, start, endt, k, pi,
+                           [&](uint32_t m, float, float, float, float d2, const float4 &) {
+            const float d = sqrtf(d2);
+            const float v = h - d;
+            rho += v * v * w2c;       // SmoothingPow2, kernels.h:31
+            rhon += v * v * v * w3c;  // SmoothingPow3, kernels.h:43
+            if (m != k) {
+                if (c < kNbrMax) my[c] = m;
+                ++c;
+            }
+        });
+        dens[k] = make_float4(rho, rhon, (rho - S.rho0) * S.k, rhon * S.k_near);
+        ncnt[i] = c;
+    }
+    cnt[t] = c;
+    __syncthreads();
+    const uint32_t i0 = i - t;  // first work index of this workgroup
+    uint4 *out = reinterpret_cast<uint4 *>(nbr + (size_t)i0 * kNbrMax);
+    for (uint32_t idx = t; idx < B * (kNbrMax / 4); idx += B) {
+        const uint32_t r = idx / (kNbrMax / 4), q = idx % (kNbrMax / 4);
+        if (i0 + r >= n || 4 * q >= min(cnt[r], kNbrMax)) continue;
+        const uint32_t *src = lds_rows + r * kRowPitch + 4 * q;
+        out[idx] = make_uint4(src[0], src[1], src[2], src[3]);
+    }
+}
+
+template <bool MULTI, bool WORK, bool LIST, bool NBR>
 __global__ __launch_bounds__(256) void k_pressure(SPH_GATHER_PARAMS, const float4 *__restrict__ dens,
                                                    const float4 *__restrict__ vs,
                                                    float4 *__restrict__ vs2)
@@ -496,9 +597,7 @@ __global__ __launch_bounds__(256) void k_pressure(SPH_GATHER_PARAMS, const float
     const float h = S.h, dw2c = S.dw2c, dw3c = S.dw3c;
     const float P = di.z, Pn = di.w;
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    gather<LIST, true, true>(S, xps, dens, start, endt, lst, k, pi,
-                             [&](uint32_t, float ox, float oy, float oz, float d2,
-                                 const float4 &dj) {
+    auto body = [&](uint32_t, float ox, float oy, float oz, float d2, const float4 &dj) {
         const float sp = (P + dj.z) * 0.5f;   // cc:410
         const float snp = (Pn + dj.w) * 0.5f; // cc:411
         const float d = sqrtf(d2);
@@ -517,7 +616,9 @@ __global__ __launch_bounds__(256) void k_pressure(SPH_GATHER_PARAMS, const float
         fx += dirx * b * snp / dj.y;    // cc:417
         fy += diry * b * snp / dj.y;
         fz += dirz * b * snp / dj.y;
-    });
+    };
+    if (NBR) gather_row(S, xps, dens, start, endt, nbr + (size_t)i * kNbrMax, ncnt[i], k, pi, body);
+    else gather<LIST, true, true>(S, xps, dens, start, endt, lst, k, pi, body);
     float4 v = vs[k];
     const float dt = S.dt, rho = di.x;
     v.x = v.x + (fx / rho) * dt;  // cc:421
@@ -526,7 +627,7 @@ __global__ __launch_bounds__(256) void k_pressure(SPH_GATHER_PARAMS, const float
     vs2[k] = v;
 }
 
-template <bool MULTI, bool WORK, bool LIST>
+template <bool MULTI, bool WORK, bool LIST, bool NBR>
 __global__ __launch_bounds__(256) void k_viscosity_integrate(SPH_GATHER_PARAMS,
                                                               const float4 *__restrict__ vs2,
                                                               float4 *__restrict__ pos,
@@ -536,15 +637,16 @@ __global__ __launch_bounds__(256) void k_viscosity_integrate(SPH_GATHER_PARAMS,
     const float4 vi = vs2[k];
     const float hh = S.hh, c6 = S.poly6c;
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    gather<LIST, true, true>(S, xps, vs2, start, endt, lst, k, pi,
-                             [&](uint32_t, float, float, float, float d2, const float4 &vj) {
+    auto body = [&](uint32_t, float, float, float, float d2, const float4 &vj) {
         const float d = sqrtf(d2);
         const float v = hh - d * d;          // kernels.h:78
         const float w = v * v * v * c6;      // kernels.h:79
         fx += (vj.x - vi.x) * w;             // cc:460
         fy += (vj.y - vi.y) * w;
         fz += (vj.z - vi.z) * w;
-    });
+    };
+    if (NBR) gather_row(S, xps, vs2, start, endt, nbr + (size_t)i * kNbrMax, ncnt[i], k, pi, body);
+    else gather<LIST, true, true>(S, xps, vs2, start, endt, lst, k, pi, body);
     const float dt = S.dt, mu = S.mu;
     float4 v = vi;
     v.x = vi.x + fx * mu * dt;  // cc:463
@@ -738,20 +840,34 @@ inline uint32_t gather_grid(uint32_t cap, int block)
         const uint32_t g_ = gather_grid(s.capacity, b_);                                      \
         const size_t lds_ = v_.list ? sizeof(uint32_t) * kListMax * (size_t)b_ : 0;           \
         const bool m_ = s.multi;                                                              \
-        if (m_ && v_.work && v_.list) KER<true, true, true><<<g_, b_, lds_, st>>>(__VA_ARGS__);      \
-        else if (m_ && v_.work) KER<true, true, false><<<g_, b_, lds_, st>>>(__VA_ARGS__);          \
-        else if (m_ && v_.list) KER<true, false, true><<<g_, b_, lds_, st>>>(__VA_ARGS__);          \
-        else if (m_) KER<true, false, false><<<g_, b_, lds_, st>>>(__VA_ARGS__);                    \
-        else if (v_.work && v_.list) KER<false, true, true><<<g_, b_, lds_, st>>>(__VA_ARGS__);     \
-        else if (v_.work) KER<false, true, false><<<g_, b_, lds_, st>>>(__VA_ARGS__);               \
-        else if (v_.list) KER<false, false, true><<<g_, b_, lds_, st>>>(__VA_ARGS__);               \
-        else KER<false, false, false><<<g_, b_, lds_, st>>>(__VA_ARGS__);                           \
+        if (c.nbr && v_.rows) {                                                               \
+            if (m_) KER<true, true, false, true><<<g_, b_, 0, st>>>(__VA_ARGS__);             \
+            else KER<false, true, false, true><<<g_, b_, 0, st>>>(__VA_ARGS__);               \
+        } else if (m_ && v_.work && v_.list)                                                  \
+            KER<true, true, true, false><<<g_, b_, lds_, st>>>(__VA_ARGS__);                  \
+        else if (m_ && v_.work) KER<true, true, false, false><<<g_, b_, lds_, st>>>(__VA_ARGS__);   \
+        else if (m_ && v_.list) KER<true, false, true, false><<<g_, b_, lds_, st>>>(__VA_ARGS__);   \
+        else if (m_) KER<true, false, false, false><<<g_, b_, lds_, st>>>(__VA_ARGS__);             \
+        else if (v_.work && v_.list) KER<false, true, true, false><<<g_, b_, lds_, st>>>(__VA_ARGS__); \
+        else if (v_.work) KER<false, true, false, false><<<g_, b_, lds_, st>>>(__VA_ARGS__);        \
+        else if (v_.list) KER<false, false, true, false><<<g_, b_, lds_, st>>>(__VA_ARGS__);        \
+        else KER<false, false, false, false><<<g_, b_, lds_, st>>>(__VA_ARGS__);                    \
     } while (0)
 
-#define SPH_GATHER_ARGS s.ed, s.sd, s.scene_of, s.work, s.rank, s.vals[buf], s.xps, s.start, s.end
+#define SPH_GATHER_ARGS                                                                \
+    s.ed, s.sd, s.scene_of, s.work, s.rank, s.vals[buf], s.xps, s.start, s.end, s.nbr, s.ncnt
 
 hipError_t launch_density(const DevState &s, const LaunchCfg &c, int buf, hipStream_t st)
 {
+    if (c.nbr) {
+        const int b = c.density.block;
+        const size_t lds = sizeof(uint32_t) * ((size_t)b * kRowPitch + b);
+        if (s.multi)
+            k_density_rows<true><<<gather_grid(s.capacity, b), b, lds, st>>>(SPH_GATHER_ARGS, s.dens);
+        else
+            k_density_rows<false><<<gather_grid(s.capacity, b), b, lds, st>>>(SPH_GATHER_ARGS, s.dens);
+        SPH_LAUNCH_CHECK();
+    }
     SPH_GATHER_LAUNCH(k_density, c.density, c.density.block, st, SPH_GATHER_ARGS, s.dens);
     SPH_LAUNCH_CHECK();
 }

@@ -36,6 +36,8 @@ struct DevState {
     uint32_t *wvals[2];  // particle ids (sort buffers); the sorted ones are the work order
     const uint32_t *work;  // = wvals[result] of the last rebuild
     uint32_t *rank;      // [capacity] sorted slot of each particle id
+    uint32_t *nbr;       // [capacity * 64] neighbour rows by work index (nbr mode)
+    uint32_t *ncnt;      // [capacity] neighbour count per row
     uint32_t capacity;       // particle slots allocated
     uint32_t table_capacity; // bucket slots allocated
 };
@@ -45,14 +47,19 @@ struct GatherVariant {
     bool work = false;  // Morton work order + XCD-contiguous workgroups
     bool list = false;  // two phases through a per-lane LDS list
     int block = 256;    // work-items per workgroup (64..256)
+    bool rows = false;  // pressure / viscosity: read the density pass's neighbour rows
 };
 
 struct LaunchCfg {
-    GatherVariant density{false, false, 256};
-    GatherVariant pressure{false, true, 128};
-    GatherVariant viscosity{false, false, 256};
+    // defaults: the autotuner's usual picks on the benchmark scenes
+    GatherVariant density{false, false, 64};
+    GatherVariant pressure{true, true, 64};
+    GatherVariant viscosity{false, false, 64};
+    // nbr mode: density (in the work order) stores neighbour rows; pressure
+    // and viscosity read them when their `rows` is set (then in the work order)
+    bool nbr = false;
     int block_stream = 256;  // threads per block of streaming kernels
-    bool needs_work() const { return density.work || pressure.work || viscosity.work; }
+    bool needs_work() const { return nbr || density.work || pressure.work || viscosity.work; }
 };
 
 // Stage A + B1: predict, cell, hash, key; vals = slot index.

@@ -67,6 +67,34 @@ class Params(ctypes.Structure):
         return d
 
 
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_CANCELLED = 0, 1, 2, 3
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("max_running", ctypes.c_uint32),
+                ("initial_capacity", ctypes.c_uint32), ("dt", ctypes.c_float),
+                ("substeps", ctypes.c_int32), ("updates_per_tick", ctypes.c_int32)]
+
+
+class TickInfo(ctypes.Structure):
+    _fields_ = [("running", ctypes.c_uint32), ("queued", ctypes.c_uint32),
+                ("admitted", ctypes.c_uint32), ("finished", ctypes.c_uint32),
+                ("particles", ctypes.c_uint64), ("ms", ctypes.c_double)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint64) for k in ("admitted", "finished", "rejected", "cancelled",
+                                               "queued", "running", "used_bytes",
+                                               "particle_updates", "ticks", "capacity")] + \
+               [(k, ctypes.c_double) for k in ("busy_s", "tick_ms_p50", "tick_ms_p99",
+                                               "request_ms_p50", "request_ms_p99", "wait_ms_p50",
+                                               "wait_ms_p99")]
+
+
+def _struct_dict(st):
+    return {k: getattr(st, k) for k, _ in st._fields_}
+
+
 def build():
     """Compile libsph_amd.so (hipcc, gfx950) with the package Makefile."""
     subprocess.run(["make", "-s", "-j8", "-C", PKG_DIR], check=True)
@@ -114,6 +142,29 @@ def lib():
         "sph_debug_predicted": (i32, [H, p]),
         "sph_debug_neighbours": (i32, [H, p, p, u64, p]),
         "sph_generate": (i32, [i32, u64, u32, p, f, f, f, p, p]),
+        "sph_server_default_config": (None, [ctypes.POINTER(ServerConfig)]),
+        "sph_server_request_bytes": (u64, [u32, u32]),
+        "sph_server_create": (i32, [ctypes.POINTER(ServerConfig), ctypes.POINTER(H)]),
+        "sph_server_destroy": (i32, [H]),
+        "sph_server_submit": (i32, [H, pp, u32, u32, p, p, u32, p]),
+        "sph_server_tick": (i32, [H, ctypes.POINTER(TickInfo)]),
+        "sph_server_status": (i32, [H, u64, p, p, p]),
+        "sph_server_fetch": (i32, [H, u64, p, p, p]),
+        "sph_server_cancel": (i32, [H, u64]),
+        "sph_server_get_stats": (i32, [H, ctypes.POINTER(ServerStats)]),
+        "sph_batch_create": (i32, [u32, p, p, p, ctypes.POINTER(H)]),
+        "sph_batch_destroy": (i32, [H]),
+        "sph_batch_upload": (i32, [H, u32, p, p]),
+        "sph_batch_set_params": (i32, [H, u32, pp]),
+        "sph_batch_step": (i32, [H, f, i32]),
+        "sph_batch_step_async": (i32, [H, f, i32]),
+        "sph_batch_sync": (i32, [H]),
+        "sph_batch_download": (i32, [H, u32, p, p, p]),
+        "sph_batch_autotune": (i32, [H, f, i32, p]),
+        "sph_batch_profile_begin": (i32, [H, u32]),
+        "sph_batch_profile_end": (i32, [H, p, p]),
+        "sph_batch_get_variants": (i32, [H, p]),
+        "sph_batch_set_variants": (i32, [H, p]),
         "sph_copy_calibration": (i32, [u64, i32, p]),
         "sph_hash_cell": (u32, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
         "sph_bucket_of": (u32, [u32, u32]),
@@ -178,6 +229,25 @@ def hash_cell(cx, cy, cz):
 def bucket_of(hash_, table):
     """GetKeyFromHash (physicsWorld.cc:513-516) as the kernels compute it."""
     return lib().sph_bucket_of(hash_, table)
+
+
+_KERNELS = ("density", "pressure", "viscosity")
+
+
+def _variants_dict(v):
+    d = {k: {"work": bool(v[4 * q]), "list": bool(v[4 * q + 1]), "block": int(v[4 * q + 2]),
+             "rows": bool(v[4 * q + 3])} for q, k in enumerate(_KERNELS)}
+    d["nbr"] = bool(v[12])
+    return d
+
+
+def _variants_array(d):
+    v = np.zeros(13, np.int32)
+    for q, k in enumerate(_KERNELS):
+        v[4 * q:4 * q + 4] = [int(d[k]["work"]), int(d[k]["list"]), int(d[k]["block"]),
+                              int(d[k].get("rows", False))]
+    v[12] = int(d.get("nbr", False))
+    return v
 
 
 class Sim:
@@ -272,17 +342,13 @@ class Sim:
 
     @property
     def variants(self):
-        v = np.zeros(9, np.int32)
+        v = np.zeros(13, np.int32)
         _check(lib().sph_get_variants(self._h, _ptr(v)))
-        return {k: {"work": bool(v[3 * q]), "list": bool(v[3 * q + 1]), "block": int(v[3 * q + 2])}
-                for q, k in enumerate(("density", "pressure", "viscosity"))}
+        return _variants_dict(v)
 
     @variants.setter
     def variants(self, d):
-        v = np.zeros(9, np.int32)
-        for q, k in enumerate(("density", "pressure", "viscosity")):
-            v[3 * q:3 * q + 3] = [int(d[k]["work"]), int(d[k]["list"]), int(d[k]["block"])]
-        _check(lib().sph_set_variants(self._h, _ptr(v)))
+        _check(lib().sph_set_variants(self._h, _ptr(_variants_array(d))))
 
     def count_nonfinite(self):
         c = np.zeros(1, np.uint64)
@@ -311,3 +377,155 @@ class Sim:
         lst = np.zeros(max(int(total[0]), 1), np.uint32)
         _check(lib().sph_debug_neighbours(self._h, _ptr(offs), _ptr(lst), lst.size, _ptr(total)))
         return offs, lst[: int(total[0])]
+
+
+class Batch:
+    """Many independent scenes stepped by one launch per kernel (sph_batch_*)."""
+
+    def __init__(self, params_list, counts, table_sizes=None):
+        self._h = ctypes.c_void_p()
+        self.counts = [int(c) for c in counts]
+        n = len(self.counts)
+        if len(params_list) != n:
+            raise ValueError("one Params per scene")
+        arr = (Params * n)(*params_list)
+        cnt = np.asarray(self.counts, np.uint32)
+        tab = None if table_sizes is None else np.asarray(table_sizes, np.uint32)
+        _check(lib().sph_batch_create(n, _ptr(cnt), _ptr(tab), ctypes.cast(arr, ctypes.c_void_p),
+                                      ctypes.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().sph_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, scene, pos4=None, vel4=None):
+        n = self.counts[scene]
+        pos = None if pos4 is None else np.ascontiguousarray(pos4, np.float32)
+        vel = None if vel4 is None else np.ascontiguousarray(vel4, np.float32)
+        for a in (pos, vel):
+            if a is not None and a.shape != (n, 4):
+                raise ValueError(f"expected shape {(n, 4)}, got {a.shape}")
+        _check(lib().sph_batch_upload(self._h, scene, _ptr(pos), _ptr(vel)))
+
+    def step(self, dt, substeps=1):
+        _check(lib().sph_batch_step(self._h, dt, substeps))
+
+    def step_async(self, dt, substeps=1):
+        _check(lib().sph_batch_step_async(self._h, dt, substeps))
+
+    def sync(self):
+        _check(lib().sph_batch_sync(self._h))
+
+    def download(self, scene):
+        n = self.counts[scene]
+        pos = np.zeros((n, 4), np.float32)
+        vel = np.zeros((n, 4), np.float32)
+        dens = np.zeros((n, 2), np.float32)
+        _check(lib().sph_batch_download(self._h, scene, _ptr(pos), _ptr(vel), _ptr(dens)))
+        return pos, vel, dens
+
+    def autotune(self, dt, updates=3):
+        ms = np.zeros(3, np.float64)
+        _check(lib().sph_batch_autotune(self._h, dt, int(updates), _ptr(ms)))
+        return ms
+
+    def profile_begin(self, max_updates):
+        _check(lib().sph_batch_profile_begin(self._h, int(max_updates)))
+
+    def profile_end(self):
+        ms = np.zeros(6, np.float64)
+        u = np.zeros(1, np.uint32)
+        _check(lib().sph_batch_profile_end(self._h, _ptr(ms), _ptr(u)))
+        return ms, int(u[0])
+
+    @property
+    def variants(self):
+        v = np.zeros(13, np.int32)
+        _check(lib().sph_batch_get_variants(self._h, _ptr(v)))
+        return _variants_dict(v)
+
+    @variants.setter
+    def variants(self, d):
+        _check(lib().sph_batch_set_variants(self._h, _ptr(_variants_array(d))))
+
+
+def request_bytes(n, table=0):
+    return int(lib().sph_server_request_bytes(int(n), int(table)))
+
+
+class Server:
+    """Continuous batching of bake requests (sph_server_*)."""
+
+    def __init__(self, **cfg):
+        c = ServerConfig()
+        lib().sph_server_default_config(ctypes.byref(c))
+        for k, v in cfg.items():
+            setattr(c, k, v)
+        self.config = c
+        self._h = ctypes.c_void_p()
+        _check(lib().sph_server_create(ctypes.byref(c), ctypes.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().sph_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, params, pos4, vel4, steps, table=0):
+        pos = np.ascontiguousarray(pos4, np.float32)
+        vel = np.ascontiguousarray(vel4, np.float32)
+        rid = np.zeros(1, np.uint64)
+        _check(lib().sph_server_submit(self._h, ctypes.byref(params), pos.shape[0], int(table),
+                                       _ptr(pos), _ptr(vel), int(steps), _ptr(rid)))
+        return int(rid[0])
+
+    def tick(self):
+        t = TickInfo()
+        _check(lib().sph_server_tick(self._h, ctypes.byref(t)))
+        return _struct_dict(t)
+
+    def status(self, rid):
+        st = np.zeros(1, np.int32)
+        done = np.zeros(1, np.uint64)
+        lat = np.zeros(1, np.float64)
+        _check(lib().sph_server_status(self._h, int(rid), _ptr(st), _ptr(done), _ptr(lat)))
+        return int(st[0]), int(done[0]), float(lat[0])
+
+    def fetch(self, rid, n):
+        pos = np.zeros((n, 4), np.float32)
+        vel = np.zeros((n, 4), np.float32)
+        dens = np.zeros((n, 2), np.float32)
+        _check(lib().sph_server_fetch(self._h, int(rid), _ptr(pos), _ptr(vel), _ptr(dens)))
+        return pos, vel, dens
+
+    def cancel(self, rid):
+        _check(lib().sph_server_cancel(self._h, int(rid)))
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().sph_server_get_stats(self._h, ctypes.byref(st)))
+        return _struct_dict(st)

@@ -132,16 +132,18 @@ int sph_profile_begin(sph_handle h, uint32_t max_updates);
 int sph_profile_end(sph_handle h, double mean_ms[6], uint32_t *updates);
 /* Autotuner (no reference counterpart): times every variant of the three
  * gather kernels (work order on/off, LDS neighbour list on/off, workgroup
- * size 64/128/256) on the current state with `updates` Update(dt) calls per
+ * size 64/128/256, and the neighbour-row mode) on the current state with `updates` Update(dt) calls per
  * candidate, restoring positions and velocities after each, and keeps the
  * fastest variant per kernel.  All variants give bit-identical results.
  * ms (may be NULL) gets the chosen variants' mean ms (density, pressure,
  * viscosity).  Densities and debug tables are recomputed by the next update. */
 int sph_autotune(sph_handle h, float dt, int updates, double ms[3]);
-/* The variant of each gather kernel: v[3q+0] work order, v[3q+1] LDS list,
- * v[3q+2] workgroup size, q = 0 density, 1 pressure, 2 viscosity. */
-int sph_get_variants(sph_handle h, int32_t v[9]);
-int sph_set_variants(sph_handle h, const int32_t v[9]);
+/* The variant of each gather kernel, q = 0 density, 1 pressure, 2 viscosity:
+ * v[4q+0] work order, v[4q+1] LDS list, v[4q+2] workgroup size (64, 128,
+ * 192, 256), v[4q+3] read neighbour rows (pressure, viscosity); v[12] row
+ * mode (the density pass stores each particle's neighbour row). */
+int sph_get_variants(sph_handle h, int32_t v[13]);
+int sph_set_variants(sph_handle h, const int32_t v[13]);
 /* Number of non-finite position/velocity components in the current state. */
 int sph_count_nonfinite(sph_handle h, uint64_t *count);
 
@@ -168,6 +170,90 @@ int sph_debug_neighbours(sph_handle h, uint64_t *offs, uint32_t *list, uint64_t 
  * GetKeyFromHash (cc:513-516), i.e. hash % table_size. ---- */
 uint32_t sph_hash_cell(int32_t cx, int32_t cy, int32_t cz);
 uint32_t sph_bucket_of(uint32_t hash, uint32_t table_size);
+
+/* ---- batching: many independent scenes per launch ----
+ * No reference counterpart (the reference simulates one scene per process,
+ * physicsWorld.cc:32-37).  A batch owns n_scenes scenes of uneven sizes, each
+ * with its own settings and table size, packed into one set of device
+ * arrays; every kernel of an update is ONE launch over all scenes (bucket keys
+ * are offset per scene so scenes never interact), and the update is replayed
+ * from a captured HIP graph.  Each scene's results are bit-identical to the
+ * same scene run alone through sph_create/sph_update.
+ * counts[s] particles and table_sizes[s] buckets (table_sizes NULL or an
+ * entry 0 = counts[s]); params[s] settings. */
+typedef struct sph_batch_t *sph_batch;
+int sph_batch_create(uint32_t n_scenes, const uint32_t *counts, const uint32_t *table_sizes,
+                     const sph_params *params, sph_batch *out);
+int sph_batch_destroy(sph_batch b);
+int sph_batch_upload(sph_batch b, uint32_t scene, const float *pos4, const float *vel4);
+int sph_batch_set_params(sph_batch b, uint32_t scene, const sph_params *p);
+/* Every scene: `substeps` Update(dt / substeps) calls; blocking / async. */
+int sph_batch_step(sph_batch b, float dt, int substeps);
+int sph_batch_step_async(sph_batch b, float dt, int substeps);
+int sph_batch_sync(sph_batch b);
+int sph_batch_download(sph_batch b, uint32_t scene, float *pos4, float *vel4, float *dens2);
+/* As sph_autotune / sph_profile_begin / sph_profile_end, over the whole batch. */
+int sph_batch_autotune(sph_batch b, float dt, int updates, double ms[3]);
+int sph_batch_profile_begin(sph_batch b, uint32_t max_updates);
+int sph_batch_profile_end(sph_batch b, double mean_ms[6], uint32_t *updates);
+int sph_batch_get_variants(sph_batch b, int32_t v[13]);
+int sph_batch_set_variants(sph_batch b, const int32_t v[13]);
+
+/* ---- serving: continuous batching of bake requests ----
+ * No reference counterpart.  A request is a scene (settings, particle count,
+ * table size, initial state) to advance by `steps` steps of `substeps`
+ * Update(dt / substeps) calls.  Requests wait in arrival order and are
+ * admitted while one of max_running slots is free and the device bytes of
+ * the running requests (sph_server_request_bytes each) stay within
+ * hbm_budget_bytes; a request larger than the budget on its own is refused
+ * at submission (SPH_ERR_BUDGET).  Each sph_server_tick admits, advances
+ * every running request by updates_per_tick Update calls -- one launch per
+ * kernel for all of them -- and retires the finished ones, whose results are
+ * kept in host memory until fetched.  Each result is bit-identical to the
+ * same scene run alone.  Calls on one server must be serialised by the
+ * caller. */
+#define SPH_REQ_QUEUED 0
+#define SPH_REQ_RUNNING 1
+#define SPH_REQ_DONE 2
+#define SPH_REQ_CANCELLED 3
+
+typedef struct sph_server_config {
+    uint64_t hbm_budget_bytes; /* device bytes of running requests (default 200 GiB) */
+    uint32_t max_running;      /* slots: requests advanced together (default 64) */
+    uint32_t initial_capacity; /* particle slots allocated at start; grows by doubling */
+    float dt;                  /* frame dt of one step (default 1/120) */
+    int32_t substeps;          /* Update calls per step (default 3) */
+    int32_t updates_per_tick;  /* Update calls per tick (default 1: re-batch every substep) */
+} sph_server_config;
+
+typedef struct sph_server_tick_info {
+    uint32_t running, queued, admitted, finished;
+    uint64_t particles; /* particles advanced by this tick */
+    double ms;          /* wall time of the tick */
+} sph_server_tick_info;
+
+typedef struct sph_server_stats {
+    uint64_t admitted, finished, rejected, cancelled, queued, running;
+    uint64_t used_bytes, particle_updates, ticks, capacity;
+    double busy_s; /* wall time of ticks that advanced requests */
+    double tick_ms_p50, tick_ms_p99, request_ms_p50, request_ms_p99, wait_ms_p50, wait_ms_p99;
+} sph_server_stats;
+
+typedef struct sph_server_t *sph_server;
+void sph_server_default_config(sph_server_config *c);
+uint64_t sph_server_request_bytes(uint32_t n, uint32_t table_size);
+int sph_server_create(const sph_server_config *cfg, sph_server *out);
+int sph_server_destroy(sph_server s);
+int sph_server_submit(sph_server s, const sph_params *p, uint32_t n, uint32_t table_size,
+                      const float *pos4, const float *vel4, uint32_t steps, uint64_t *id);
+int sph_server_tick(sph_server s, sph_server_tick_info *info);
+int sph_server_status(sph_server s, uint64_t id, int32_t *state, uint64_t *updates_done,
+                      double *latency_ms);
+/* Copies a finished request's positions, velocities and densities (any may be
+ * NULL) and forgets the request. */
+int sph_server_fetch(sph_server s, uint64_t id, float *pos4, float *vel4, float *dens2);
+int sph_server_cancel(sph_server s, uint64_t id);
+int sph_server_get_stats(sph_server s, sph_server_stats *st);
 
 /* ---- synthetic scenes (host only; identical bytes for every consumer) ---- */
 #define SPH_INIT_GRID_ARRANGEMENT 0 /* the reference's InitializeData lattice, cc:518-557 */

@@ -167,7 +167,10 @@ def test_initialize_data_matches_reference_lattice():
     assert_bitexact(dens_g, O.density(op, pos_o, sp), "InitializeData densities")
 
 
-VARIANTS = [(w, l, b) for w in (0, 1) for l in (0, 1) for b in (64, 128, 256)]
+# (work, list, block, row mode); in row mode pressure reads rows and
+# viscosity alternates between rows and its walk
+VARIANTS = [(w, l, b, 0) for w in (0, 1) for l in (0, 1) for b in (64, 128, 256)] + \
+    [(1, 0, b, 1) for b in (64, 128, 256)]
 
 
 @pytest.mark.parametrize("case", ["dam_break_16k", "uniform_64k_T1000", "dense_lattice_overflow"])
@@ -177,10 +180,12 @@ def test_every_gather_variant_bitexact(case):
     dt = _sub_dt()
     ref = _oracle_update(sc, dt)
     with S.Sim(sc.params, sc.n, sc.table) as sim:
-        for (w, l, b) in VARIANTS:
-            announce("variant", case, "work", w, "list", l, "block", b)
-            v = {"work": w, "list": l, "block": b}
-            sim.variants = {"density": v, "pressure": v, "viscosity": v}
+        for (w, l, b, nb) in VARIANTS:
+            announce("variant", case, "work", w, "list", l, "block", b, "nbr", nb)
+            v = {"work": w, "list": l, "block": b, "rows": bool(nb)}
+            vv = dict(v, rows=bool(nb) and b != 128)
+            sim.variants = {"density": v, "pressure": v, "viscosity": vv, "nbr": nb}
+            assert sim.variants["nbr"] == bool(nb)
             sim.upload(sc.pos, sc.vel)
             sim.update(dt)
             pos_g, vel_g, dens_g = sim.download()
