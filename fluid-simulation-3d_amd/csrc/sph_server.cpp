@@ -86,6 +86,13 @@ struct sph_server_t {
     uint64_t particle_updates = 0;      // particles x updates advanced
     double busy_s = 0;                  // wall time spent in ticks with work
     uint64_t finished = 0, admitted = 0, rejected = 0, cancelled = 0;
+    // GPU timing of ticks: event pairs in a ring; at most kInFlight ticks are
+    // enqueued ahead of the GPU, so host-side admission overlaps device work
+    static constexpr int kRing = 8;
+    static constexpr int kInFlight = 2;
+    hipEvent_t ev_start[kRing] = {}, ev_end[kRing] = {};
+    std::deque<int> pending;  // ring slots of enqueued, not yet harvested ticks
+    uint64_t tick_no = 0;
 };
 
 namespace {
@@ -231,6 +238,12 @@ int sph_server_create(const sph_server_config *cfg, sph_server *out)
         return fail(SPH_ERR_HIP, "no HIP device visible");
     sph_server_t *s = new sph_server_t();
     s->cfg = *cfg;
+    for (int i = 0; i < sph_server_t::kRing; ++i)
+        if (hipEventCreate(&s->ev_start[i]) != hipSuccess ||
+            hipEventCreate(&s->ev_end[i]) != hipSuccess) {
+            sph_server_destroy(s);
+            return fail(SPH_ERR_HIP, "event creation failed");
+        }
     std::string err;
     int rc = ensure_capacity(s, 1, 1, &err);
     if (rc) {
@@ -246,6 +259,11 @@ int sph_server_destroy(sph_server s)
 {
     SRV_GUARD_BEGIN
     if (!s) return SPH_OK;
+    if (s->eng) (void)hipStreamSynchronize(s->eng->stream());
+    for (int i = 0; i < sph_server_t::kRing; ++i) {
+        if (s->ev_start[i]) (void)hipEventDestroy(s->ev_start[i]);
+        if (s->ev_end[i]) (void)hipEventDestroy(s->ev_end[i]);
+    }
     delete s->eng;
     delete s;
     return SPH_OK;
@@ -310,13 +328,41 @@ int sph_server_tick(sph_server s, sph_server_tick_info *info)
     uint32_t finished = 0;
     uint64_t particles = 0;
     const float sub = s->cfg.dt / (float)s->cfg.substeps;
+    // harvest the GPU durations of completed ticks
+    auto harvest = [&](bool wait) -> int {
+        while (!s->pending.empty()) {
+            const int slot = s->pending.front();
+            hipError_t q = wait ? hipEventSynchronize(s->ev_end[slot]) : hipEventQuery(s->ev_end[slot]);
+            if (q == hipErrorNotReady) return SPH_OK;
+            if (q != hipSuccess) return fail(SPH_ERR_HIP, "tick event");
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, s->ev_start[slot], s->ev_end[slot]) != hipSuccess)
+                return fail(SPH_ERR_HIP, "tick timer");
+            s->tick_ms.push_back(ms);
+            s->busy_s += ms * 1e-3;
+            s->pending.pop_front();
+            wait = false;
+        }
+        return SPH_OK;
+    };
     if (!s->running.empty()) {
+        // keep at most kInFlight ticks ahead of the device
+        while ((int)s->pending.size() >= sph_server_t::kInFlight) {
+            const int slot = s->pending.front();
+            if (hipEventSynchronize(s->ev_end[slot]) != hipSuccess) return fail(SPH_ERR_HIP, "tick wait");
+            int hr = harvest(false);
+            if (hr) return hr;
+        }
+        const int slot = (int)(s->tick_no++ % sph_server_t::kRing);
+        hipStream_t st = s->eng->stream();
+        if (hipEventRecord(s->ev_start[slot], st) != hipSuccess) return fail(SPH_ERR_HIP, "tick event");
         for (int u = 0; u < s->cfg.updates_per_tick; ++u) {
             int rc = s->eng->update(sub, &err);
             if (rc) return fail(rc, err);
         }
-        int rc = s->eng->sync(&err);
-        if (rc) return fail(rc, err);
+        if (hipEventRecord(s->ev_end[slot], st) != hipSuccess) return fail(SPH_ERR_HIP, "tick event");
+        s->pending.push_back(slot);
+        int rc = SPH_OK;
         // 3. retire finished requests: results to host, slot freed at the next repack
         for (size_t q = 0; q < s->running.size(); ++q) {
             Request &r = s->reqs.at(s->running[q]);
@@ -330,6 +376,7 @@ int sph_server_tick(sph_server s, sph_server_tick_info *info)
                 r.pos.resize(4 * (size_t)r.n);
                 r.vel.resize(4 * (size_t)r.n);
                 r.dens.resize(2 * (size_t)r.n);
+                // (download waits for the enqueued ticks)
                 rc = s->eng->download((uint32_t)q, r.pos.data(), r.vel.data(), r.dens.data(), &err);
                 if (rc) return fail(rc, err);
                 r.state = SPH_REQ_DONE;
@@ -342,9 +389,10 @@ int sph_server_tick(sph_server s, sph_server_tick_info *info)
                 ++s->finished;
             }
         }
-        const double ms = (now_s() - t0) * 1e3;
-        s->tick_ms.push_back(ms);
-        s->busy_s += ms * 1e-3;
+    }
+    {
+        int hr = harvest(false);
+        if (hr) return hr;
     }
     if (info) {
         uint32_t run = 0;
@@ -433,6 +481,17 @@ int sph_server_get_stats(sph_server s, sph_server_stats *st)
 {
     SRV_GUARD_BEGIN
     if (!s || !st) return fail(SPH_ERR_INVALID, "null argument");
+    // finish timing the enqueued ticks
+    while (!s->pending.empty()) {
+        const int slot = s->pending.front();
+        float ms = 0.0f;
+        if (hipEventSynchronize(s->ev_end[slot]) != hipSuccess ||
+            hipEventElapsedTime(&ms, s->ev_start[slot], s->ev_end[slot]) != hipSuccess)
+            return fail(SPH_ERR_HIP, "tick timer");
+        s->tick_ms.push_back(ms);
+        s->busy_s += ms * 1e-3;
+        s->pending.pop_front();
+    }
     memset(st, 0, sizeof(*st));
     st->admitted = s->admitted;
     st->finished = s->finished;
