@@ -67,6 +67,9 @@ void fill_scene_desc(const sph_params &p, uint32_t n, uint32_t table, SceneDesc 
     d->h = r;
     d->r2 = p.prefilter_r2;
     d->h2_lt = sqrt_threshold(r);
+    // one comparison for both tests of the walk (prefilter d2 <= r2, cc:357,
+    // and the kernels' support d < h): valid for every non-NaN d2
+    d->thr = fminf(d->h2_lt, nextafterf(p.prefilter_r2, INFINITY));
     d->rho0 = p.target_density;
     d->k = p.pressure_mult;
     d->k_near = p.near_pressure_mult;

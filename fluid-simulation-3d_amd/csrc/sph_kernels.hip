@@ -309,7 +309,7 @@ __device__ __forceinline__ void walk(const SceneDesc &S, const float4 *__restric
                                      const float4 &pi, Body &&body)
 {
     const int cx = cell_coord(pi.x, S.h), cy = cell_coord(pi.y, S.h), cz = cell_coord(pi.z, S.h);
-    const float r2 = S.r2, h2lt = S.h2_lt;
+    const float thr = S.thr;
     const uint32_t key_off = S.key_off, table = S.table;
     const uint64_t mod_m = S.mod_m;
     uint32_t nb, ne;
@@ -345,8 +345,7 @@ __device__ __forceinline__ void walk(const SceneDesc &S, const float4 *__restric
                 oy[j] = r[j].y - pi.y;
                 oz[j] = r[j].z - pi.z;
                 d2[j] = ox[j] * ox[j] + oy[j] * oy[j] + oz[j] * oz[j];  // glm::dot order
-                pass[j] = r[j].w == hf && !(SKIP_SELF && m + j == k) && !(d2[j] > r2) &&
-                          d2[j] < h2lt;
+                pass[j] = r[j].w == hf && !(SKIP_SELF && m + j == k) && d2[j] < thr;
             }
             float4 a[KB];
 #pragma unroll

@@ -23,6 +23,7 @@ struct SceneDesc {
     float h;              // interactionRadius
     float r2;             // distance prefilter sqrRadius (physicsWorld.h:96)
     float h2_lt;          // smallest float x with sqrtf(x) >= h: d < h  <=>  d2 < h2_lt
+    float thr;            // min(h2_lt, next float above r2): d2 <= r2 && d < h  <=>  d2 < thr
     float rho0, k, k_near, mu;
     float w2c, w3c;       // SmoothingPow2/Pow3 volumes (kernels.h:29, 41)
     float dw2c, dw3c;     // derivative scales (kernels.h:53, 65)
@@ -34,7 +35,7 @@ struct SceneDesc {
     float damp_neg;       // -1 * dampFactor (cc:94)
     float dt;             // substep dt of the current Update
     int32_t cell_lo[3];   // origin of the scene's cell grid for the work-order key
-    uint32_t pad[2];
+    uint32_t pad[1];
 };
 
 // Whole-engine counters, resident in device memory so that a captured graph
