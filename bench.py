@@ -37,6 +37,37 @@ HBM_PEAK_GBS = 8000.0
 #   viscosity+integrate: R xp, v, slot (36), R+W pos (32), W vel (16)
 ALG_BYTES = {"predict": 72.0, "density": 32.0, "pressure": 64.0, "viscosity_integrate": 84.0}
 STAGES = ["predict", "spatial", "density", "pressure", "viscosity_integrate"]
+# Per-dispatch HBM traffic of each kernel instantiation on uniform_1m, from two
+# rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of this script, each scaled by
+# what the same counter reads for a device copy of known size.
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "uniform_1m_traffic.json")
+
+
+def kernel_name(stage, variants, multi=False):
+    """Template instantiation that runs `stage` under the gather variants `variants`."""
+    m = "true" if multi else "false"
+    b = lambda x: "true" if x else "false"  # noqa: E731
+    if stage == "density" and variants["nbr"]:
+        return f"k_density_rows<{m}>"
+    base = {"density": "k_density", "pressure": "k_pressure",
+            "viscosity_integrate": "k_viscosity_integrate"}[stage]
+    v = variants[stage.split("_")[0]]
+    if variants["nbr"] and v["rows"]:
+        return f"{base}<{m}, true, false, true>"
+    return f"{base}<{m}, {b(v['work'])}, {b(v['list'])}, false>"
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary, or None."""
+    try:
+        with open(TRAFFIC_JSON) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("workload") != workload:
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    return None if k is None else float(k["traffic_bytes"])
 
 
 def log(*a):
@@ -186,7 +217,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="uniform_1m")
     ap.add_argument("--latency-steps", type=int, default=100)
-    ap.add_argument("--cpu-substeps", type=int, default=3)
+    ap.add_argument("--cpu-substeps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
@@ -254,6 +285,7 @@ def main():
     gather = ["density", "pressure", "viscosity_integrate"]
     dom = max(gather, key=lambda k: per[k])
     ach = ALG_BYTES[dom] * sc.n / (per[dom] * 1e-3) / 1e9
+    dom_kernel = kernel_name(dom, variants)
     out = {
         "metric": "particle-steps/s (single scene, 1 MI355X)",
         "value": value,
@@ -274,8 +306,10 @@ def main():
                             "p99": float(np.percentile(lat, 99)), "n": int(lat.size)},
         "stage_ms_per_update": per,
         "gather_variants": variants,
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+        "roofline": {"bound": "hbm", "kernel": dom_kernel, "achieved": ach, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(sc.name, dom_kernel),
+                     "alg_bytes_per_launch": ALG_BYTES[dom] * sc.n,
                      "alg_bytes_per_particle": ALG_BYTES[dom]},
         "nonfinite_components": nonfinite,
     }
