@@ -43,18 +43,21 @@ STAGES = ["predict", "spatial", "density", "pressure", "viscosity_integrate"]
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "uniform_1m_traffic.json")
 
 
-def kernel_name(stage, variants, multi=False):
-    """Template instantiation that runs `stage` under the gather variants `variants`."""
+def kernel_name(stage, variants, grid, multi=False):
+    """Template instantiation that runs `stage` under the gather variants `variants`
+    (kernel<MULTI, WORK, LIST, NBR, GRID>)."""
     m = "true" if multi else "false"
     b = lambda x: "true" if x else "false"  # noqa: E731
-    if stage == "density" and variants["nbr"]:
+    if stage == "density" and variants["nbr"] and not grid:
         return f"k_density_rows<{m}>"
     base = {"density": "k_density", "pressure": "k_pressure",
             "viscosity_integrate": "k_viscosity_integrate"}[stage]
     v = variants[stage.split("_")[0]]
+    if grid:
+        return f"{base}<{m}, false, {b(v['list'])}, false, true>"
     if variants["nbr"] and v["rows"]:
-        return f"{base}<{m}, true, false, true>"
-    return f"{base}<{m}, {b(v['work'])}, {b(v['list'])}, false>"
+        return f"{base}<{m}, true, false, true, false>"
+    return f"{base}<{m}, {b(v['work'])}, {b(v['list'])}, false, false>"
 
 
 def measured_traffic(workload, kernel):
@@ -97,15 +100,16 @@ def copy_calibration(mib):
         f"({2 * (mib << 20) / (ms * 1e-3) / 1e9:.0f} GB/s read+write)")
 
 
-def bench_batch(steps, plain_steps, warmup):
+def bench_batch(steps, plain_steps, warmup, key="grid"):
     """Batched (one launch per kernel for all scenes, graph-replayed) vs plain
     (one handle per scene, each kernel launch covering one scene, all handles
     enqueued asynchronously) on the same 256-scene workload."""
     bs = W.batch()
     total = bs.total
     log(f"[bench] batch: {len(bs.scenes)} scenes, {total} particles")
+    grid = key == "grid"
     b = S.Batch([sc.params for sc in bs.scenes], [sc.n for sc in bs.scenes],
-                [sc.table for sc in bs.scenes])
+                None if grid else [sc.table for sc in bs.scenes])
     for s_, sc in enumerate(bs.scenes):
         b.upload(s_, sc.pos, sc.vel)
     for _ in range(warmup):
@@ -133,7 +137,7 @@ def bench_batch(steps, plain_steps, warmup):
 
     sims = []
     for sc in bs.scenes:
-        sim = S.Sim(sc.params, sc.n, sc.table)
+        sim = S.Sim(sc.params, sc.n, S.TABLE_GRID if grid else sc.table)
         sim.upload(sc.pos, sc.vel)
         sims.append(sim)
     for sim in sims:
@@ -153,7 +157,7 @@ def bench_batch(steps, plain_steps, warmup):
     v_b = total * steps / el_b
     v_p = total * plain_steps / el_p
     return {"workload": bs.name, "scenes": len(bs.scenes), "particles": total,
-            "value": v_b, "unit": "particle-steps/s", "ms_per_step": el_b * 1e3 / steps,
+            "bucket_key": key, "value": v_b, "unit": "particle-steps/s", "ms_per_step": el_b * 1e3 / steps,
             "steps": steps,
             "plain_one_scene_per_launch": {"value": v_p, "ms_per_step": el_p * 1e3 / plain_steps,
                                            "steps": plain_steps},
@@ -216,6 +220,9 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="uniform_1m")
+    ap.add_argument("--key", choices=["grid", "reference"], default="grid",
+                    help="bucket key of the spatial lookup (results are identical): the "
+                         "cell-grid key, or the reference's hash %% T with the workload's T")
     ap.add_argument("--latency-steps", type=int, default=100)
     ap.add_argument("--cpu-substeps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
@@ -244,7 +251,8 @@ def main():
         copy_calibration(args.copy_calibration_mib)
     sc = W.by_name(args.workload)
     log(f"[bench] workload {sc.name} n={sc.n} table={sc.table} steps={args.steps} warmup={args.warmup}")
-    sim = S.Sim(sc.params, sc.n, sc.table)
+    sim = S.Sim(sc.params, sc.n, S.TABLE_GRID if args.key == "grid" else sc.table)
+    key_grid, key_gdim, _ = sim.key()
     sim.upload(sc.pos, sc.vel)
     for _ in range(args.warmup):
         sim.step_async(sc.dt, sc.substeps)
@@ -277,6 +285,7 @@ def main():
         lat.append((time.perf_counter() - t) * 1e3)
     lat = np.array(lat)
     nonfinite = sim.count_nonfinite()
+    sim_table = sim.table
     variants = sim.variants
     sim.close()
 
@@ -285,7 +294,7 @@ def main():
     gather = ["density", "pressure", "viscosity_integrate"]
     dom = max(gather, key=lambda k: per[k])
     ach = ALG_BYTES[dom] * sc.n / (per[dom] * 1e-3) / 1e9
-    dom_kernel = kernel_name(dom, variants)
+    dom_kernel = kernel_name(dom, variants, key_grid)
     out = {
         "metric": "particle-steps/s (single scene, 1 MI355X)",
         "value": value,
@@ -299,7 +308,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": sc.name, "particles": sc.n, "table_size": sc.table,
+        "config": {"workload": sc.name, "particles": sc.n, "reference_table_size": sc.table,
+                   "bucket_key": ("cell grid %dx%dx%d" % tuple(key_gdim)) if key_grid
+                   else "reference hash %% %d" % sc.table, "buckets": sim_table,
                    "h": sc.params.smoothing_radius, "dt": sc.dt, "substeps": sc.substeps,
                    "parallelism": "single GPU"},
         "step_latency_ms": {"p50": float(np.percentile(lat, 50)),
@@ -314,7 +325,7 @@ def main():
         "nonfinite_components": nonfinite,
     }
     if not args.no_batch:
-        out["batch"] = bench_batch(args.batch_steps, args.plain_steps, 3)
+        out["batch"] = bench_batch(args.batch_steps, args.plain_steps, 3, args.key)
     if not args.no_serve:
         out["serve"] = {"workload": "serving (Poisson bake requests, 8,192-2,097,152 particles, "
                                     "100-2,000 steps)",

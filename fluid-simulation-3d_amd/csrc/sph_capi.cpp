@@ -138,10 +138,14 @@ int sph_create(const sph_params *p, uint32_t n, uint32_t table_size, sph_handle 
     *out = nullptr;
     if (n == 0) return fail(SPH_ERR_INVALID, "n must be > 0");
     if (!have_gpu()) return fail(SPH_ERR_HIP, "no HIP device visible");
-    sph::SceneSpec spec{*p, n, table_size ? table_size : n};
+    if (const char *m = sph::check_params(*p)) return fail(SPH_ERR_INVALID, m);
+    const bool grid = table_size == SPH_TABLE_GRID;
+    const uint32_t t = grid ? sph::grid_table_size(*p, n)
+                            : (table_size == SPH_TABLE_REFERENCE ? n : table_size);
+    sph::SceneSpec spec{*p, n, t};
     std::string err;
     sph::Engine *e = nullptr;
-    int rc = sph::Engine::create({spec}, n, spec.table, 1, &e, &err);
+    int rc = sph::Engine::create({spec}, n, spec.table, 1, &e, &err, false, grid);
     if (rc) return fail(rc, err);
     *out = new sph_sim{e};
     return SPH_OK;
@@ -181,6 +185,18 @@ int sph_get_size(sph_handle h, uint32_t *n, uint32_t *table_size)
     SPH_NEED(h);
     if (n) *n = h->eng->scene(0).n;
     if (table_size) *table_size = h->eng->scene(0).table;
+    return SPH_OK;
+}
+
+int sph_get_key(sph_handle h, int32_t *grid, uint32_t gdim[3], int32_t lo[3])
+{
+    SPH_NEED(h);
+    const SceneDesc &d = h->eng->scene(0);
+    if (grid) *grid = (int32_t)d.grid;
+    for (int c = 0; c < 3; ++c) {
+        if (gdim) gdim[c] = d.gdim[c];
+        if (lo) lo[c] = d.cell_lo[c];
+    }
     return SPH_OK;
 }
 
@@ -361,13 +377,18 @@ int sph_batch_create(uint32_t n_scenes, const uint32_t *counts, const uint32_t *
     *out = nullptr;
     if (!have_gpu()) return fail(SPH_ERR_HIP, "no HIP device visible");
     std::vector<sph::SceneSpec> specs(n_scenes);
+    const bool grid = table_sizes == nullptr;
     for (uint32_t s = 0; s < n_scenes; ++s) {
-        const uint32_t t = table_sizes && table_sizes[s] ? table_sizes[s] : counts[s];
+        if (const char *m = sph::check_params(params[s])) return fail(SPH_ERR_INVALID, m);
+        uint32_t t;
+        if (grid) t = sph::grid_table_size(params[s], counts[s]);
+        else if (table_sizes[s] == 0 || table_sizes[s] == SPH_TABLE_REFERENCE) t = counts[s];
+        else t = table_sizes[s];
         specs[s] = sph::SceneSpec{params[s], counts[s], t};
     }
     std::string err;
     sph::Engine *e = nullptr;
-    int rc = sph::Engine::create(specs, 0, 0, n_scenes, &e, &err);
+    int rc = sph::Engine::create(specs, 0, 0, n_scenes, &e, &err, false, grid);
     if (rc) return fail(rc, err);
     *out = new sph_batch_t{e};
     return SPH_OK;

@@ -38,6 +38,19 @@ SPH_HD uint32_t fastmod(uint32_t a, uint64_t m, uint32_t table)
     return (uint32_t)(((uint64_t)hi * table + lo_hi) >> 32);
 }
 
+// Cell-grid bucket key, per axis: w = (u32(c) - u32(lo)) mod G, with
+// (u32(c) - u32(lo)) wrapping mod 2^32.  For c in [lo, lo + G) this is c - lo.
+SPH_HD uint32_t grid_wrap(uint32_t u, uint32_t g, uint64_t m) { return u < g ? u : fastmod(u, m, g); }
+
+SPH_HD uint32_t grid_key(int32_t cx, int32_t cy, int32_t cz, const int32_t lo[3],
+                         const uint32_t g[3], const uint64_t m[3])
+{
+    const uint32_t wx = grid_wrap((uint32_t)cx - (uint32_t)lo[0], g[0], m[0]);
+    const uint32_t wy = grid_wrap((uint32_t)cy - (uint32_t)lo[1], g[1], m[1]);
+    const uint32_t wz = grid_wrap((uint32_t)cz - (uint32_t)lo[2], g[2], m[2]);
+    return (wx * g[1] + wy) * g[2] + wz;
+}
+
 // glm::sign for float (func_common.inl:144-150): (0 < x) - (x < 0).
 SPH_HD float sign_f(float x) { return (float)(0.0f < x) - (float)(x < 0.0f); }
 

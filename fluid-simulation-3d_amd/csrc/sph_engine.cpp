@@ -58,7 +58,70 @@ const char *check_params(const sph_params &p)
     return nullptr;
 }
 
-void fill_scene_desc(const sph_params &p, uint32_t n, uint32_t table, SceneDesc *d)
+namespace {
+// Cell range [lo, hi] of the box [-half, half] along one axis, 4 cells of
+// margin each side (predicted positions leave the box by |v| / 120), clamped
+// far inside the int32 range.
+void box_cells(float half, float h, int32_t *lo, int32_t *hi)
+{
+    const double lim = (double)(1 << 29);
+    const double a = std::max(-lim, std::min(lim, floor((double)(-half / h))));
+    const double b = std::max(-lim, std::min(lim, floor((double)(half / h))));
+    *lo = (int32_t)a - 4;
+    *hi = (int32_t)b + 4;
+}
+
+uint64_t box_cell_count(const sph_params &p)
+{
+    uint64_t c = 1;
+    for (int a = 0; a < 3; ++a) {
+        int32_t lo, hi;
+        box_cells(p.bounds[a] * 0.5f, p.smoothing_radius, &lo, &hi);
+        c *= (uint64_t)(hi - lo + 1);
+        if (c > (1ull << 40)) c = 1ull << 40;
+    }
+    return c;
+}
+}  // namespace
+
+uint32_t grid_table_size(const sph_params &p, uint32_t n)
+{
+    const uint64_t cap = std::min<uint64_t>(std::max<uint64_t>(8ull * n, 1ull << 20), 1ull << 28);
+    return (uint32_t)std::max<uint64_t>(1, std::min(box_cell_count(p), cap));
+}
+
+// Periods of the cell grid within `table` buckets: the box (+margin) when it
+// fits, else every axis shrunk by the same factor (cells then wrap; the
+// float-hash test of the walk keeps results exact).
+void grid_dims(const sph_params &p, uint32_t table, uint32_t g[3])
+{
+    uint64_t ideal[3], prod = 1;
+    for (int a = 0; a < 3; ++a) {
+        int32_t l, hi;
+        box_cells(p.bounds[a] * 0.5f, p.smoothing_radius, &l, &hi);
+        ideal[a] = (uint64_t)(hi - l + 1);
+        prod *= ideal[a];
+    }
+    if (prod <= table) {
+        for (int a = 0; a < 3; ++a) g[a] = (uint32_t)ideal[a];
+        return;
+    }
+    const double f = cbrt((double)table / (double)prod);
+    uint64_t q = 1;
+    for (int a = 0; a < 3; ++a) {
+        g[a] = (uint32_t)std::max<double>(1.0, floor((double)ideal[a] * f));
+        q *= g[a];
+    }
+    while (q > table) {
+        int big = 0;
+        for (int a = 1; a < 3; ++a)
+            if (g[a] > g[big]) big = a;
+        q = q / g[big] * (g[big] - 1);
+        --g[big];
+    }
+}
+
+void fill_scene_desc(const sph_params &p, uint32_t n, uint32_t table, bool grid, SceneDesc *d)
 {
     const float r = p.smoothing_radius;
     d->n = n;
@@ -86,7 +149,17 @@ void fill_scene_desc(const sph_params &p, uint32_t n, uint32_t table, SceneDesc 
     d->lookahead = p.lookahead;                          // cc:47
     for (int c = 0; c < 3; ++c) d->half[c] = p.bounds[c] * 0.5f;  // cc:88
     d->damp_neg = -1 * p.damp;                           // cc:94
-    for (int c = 0; c < 3; ++c) d->cell_lo[c] = (int)floorf(-d->half[c] / r) - 4;
+    for (int c = 0; c < 3; ++c) {
+        int32_t hi;
+        box_cells(d->half[c], r, &d->cell_lo[c], &hi);
+    }
+    d->grid = grid ? 1u : 0u;
+    if (grid) {
+        grid_dims(p, table, d->gdim);
+    } else {
+        d->gdim[0] = d->gdim[1] = d->gdim[2] = 1;
+    }
+    for (int c = 0; c < 3; ++c) d->gmod[c] = fastmod_constant(d->gdim[c]);
 }
 
 void work_key_layout(uint32_t n_scenes, EngineDesc *e)
@@ -116,7 +189,7 @@ int Engine::check(hipError_t e, const char *what, std::string *err)
 
 int Engine::create(const std::vector<SceneSpec> &scenes, uint32_t capacity,
                    uint32_t table_capacity, uint32_t scene_capacity, Engine **out,
-                   std::string *err, bool multi)
+                   std::string *err, bool multi, bool grid)
 {
     if (scenes.empty() && (capacity == 0 || table_capacity == 0 || scene_capacity == 0)) {
         *err = "no scenes and no capacities";
@@ -155,7 +228,7 @@ int Engine::create(const std::vector<SceneSpec> &scenes, uint32_t capacity,
     std::vector<uint32_t> scene_of(capacity, 0);
     for (uint32_t s = 0; s < scenes.size(); ++s) {
         SceneDesc d{};
-        fill_scene_desc(scenes[s].params, scenes[s].n, scenes[s].table, &d);
+        fill_scene_desc(scenes[s].params, scenes[s].n, scenes[s].table, grid, &d);
         d.p_off = p_off;
         d.key_off = k_off;
         d.dt = 0.0f;
@@ -172,6 +245,9 @@ int Engine::create(const std::vector<SceneSpec> &scenes, uint32_t capacity,
     e->ed_h_->key_bits = e->key_bits_;
     work_key_layout(e->ed_h_->n_scenes, e->ed_h_);
     e->d_.multi = multi || scenes.size() > 1;
+    e->grid_ = grid;
+    e->d_.grid = grid;
+    e->cfg_ = default_launch_cfg(grid);
     rc = e->check(hipMemcpy(e->d_scene_of_, scene_of.data(), sizeof(uint32_t) * capacity,
                             hipMemcpyHostToDevice),
                   "upload scene map", err);
@@ -363,7 +439,7 @@ int Engine::repack(const std::vector<RepackItem> &items, std::string *err)
     uint32_t k_off = 0;
     for (size_t q = 0; q < items.size(); ++q) {
         const RepackItem &it = items[q];
-        fill_scene_desc(it.spec.params, it.spec.n, it.spec.table, &sd[q]);
+        fill_scene_desc(it.spec.params, it.spec.n, it.spec.table, grid_, &sd[q]);
         sd[q].p_off = new_off[q];
         sd[q].key_off = k_off;
         sd[q].dt = dt_cached_ < 0.0f ? 0.0f : dt_cached_;
@@ -410,7 +486,7 @@ int Engine::set_params(uint32_t s, const sph_params &p, std::string *err)
     SceneDesc &d = sd_h_[s];
     const float dt = d.dt;
     const uint32_t p_off = d.p_off, k_off = d.key_off;
-    fill_scene_desc(p, d.n, d.table, &d);
+    fill_scene_desc(p, d.n, d.table, grid_, &d);
     d.p_off = p_off;
     d.key_off = k_off;
     d.dt = dt;
@@ -545,10 +621,12 @@ int Engine::autotune(float dt, int updates, double ms_out[3], std::string *err)
         bool nbr;
     };
     std::vector<Cand> cands;
-    for (int w = 0; w < 2; ++w)
+    // (the cell-grid key has neither the work order nor the row mode)
+    for (int w = 0; w < (grid_ ? 1 : 2); ++w)
         for (int l = 0; l < 2; ++l)
             for (int b : {64, 128, 256}) cands.push_back({GatherVariant{w != 0, l != 0, b, false}, false});
-    for (int b : {64, 128, 256}) cands.push_back({GatherVariant{true, false, b, true}, true});
+    if (!grid_)
+        for (int b : {64, 128, 256}) cands.push_back({GatherVariant{true, false, b, true}, true});
     double walk_ms[3] = {1e30, 1e30, 1e30}, row_ms[3] = {1e30, 1e30, 1e30};
     GatherVariant walk_v[3] = {keep.density, keep.pressure, keep.viscosity}, row_v[3];
     int rc = check(save(), "autotune snapshot", err);

@@ -30,7 +30,12 @@ struct RepackItem {
 
 // Fills the derived constants of a SceneDesc from user settings, with the
 // reference's float expressions (kernels.h:25-82, physicsWorld.cc:47,87-88).
-void fill_scene_desc(const sph_params &p, uint32_t n, uint32_t table, SceneDesc *d);
+// grid: bucket by the cell-grid key (periods chosen within `table` buckets)
+// instead of hash % table.
+void fill_scene_desc(const sph_params &p, uint32_t n, uint32_t table, bool grid, SceneDesc *d);
+// Bucket count of the cell-grid key for a scene: the box (+4 cells of margin
+// per side) in cells, capped at max(8n, 2^20) and 2^28.
+uint32_t grid_table_size(const sph_params &p, uint32_t n);
 // Smallest float x >= 0 with sqrtf(x) >= h (so d < h <=> d*d-as-float < x).
 float sqrt_threshold(float h);
 // Validates settings; returns an error message or nullptr.
@@ -44,7 +49,7 @@ class Engine {
     // allowed with explicit capacities (a server starts empty).
     static int create(const std::vector<SceneSpec> &scenes, uint32_t capacity,
                       uint32_t table_capacity, uint32_t scene_capacity, Engine **out,
-                      std::string *err, bool multi = false);
+                      std::string *err, bool multi = false, bool grid = false);
     // Replaces the scene list in place: buffers, grids and captured graphs stay
     // valid (the kernels read the counts from device memory).  Fails without
     // changes if the items exceed the capacities.
@@ -86,11 +91,14 @@ class Engine {
     int profile_end(double mean_ms[6], uint32_t *updates, std::string *err);
     const double *stage_ms() const { return stage_ms_; }
     const LaunchCfg &launch_cfg() const { return cfg_; }
+    // Variants that do not exist under the engine's key are turned off
+    // (the cell-grid key runs in sorted order: no work order, no rows).
     void set_launch_cfg(const LaunchCfg &c)
     {
-        cfg_ = c;
+        cfg_ = sanitize_launch_cfg(c, grid_);
         invalidate_graph();
     }
+    bool grid() const { return grid_; }
     // Times every gather-kernel variant on the current state (`updates`
     // Update(dt) calls per candidate, from a snapshot of positions and
     // velocities that is restored afterwards) and keeps the fastest variant of
@@ -132,6 +140,7 @@ class Engine {
     uint32_t scene_cap_ = 1;
     int sorted_buf_ = 0;
     bool have_tables_ = false;
+    bool grid_ = false;  // cell-grid bucket key (else the reference's hash % table)
     bool use_graph_ = true;
     hipGraph_t graph_[2] = {nullptr, nullptr};          // [rebuild work order?]
     hipGraphExec_t graph_exec_[2] = {nullptr, nullptr};

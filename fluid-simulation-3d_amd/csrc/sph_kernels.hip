@@ -107,7 +107,9 @@ __global__ void k_predict(const EngineDesc *__restrict__ ed, const SceneDesc *__
         const uint32_t hash = hash_cell(cx, cy, cz);
         vel[g] = v;
         xp[g] = make_float4(px, py, pz, __uint_as_float(hash));
-        keys[g] = S.key_off + fastmod(hash, S.mod_m, S.table);
+        // bucket: GetKeyFromHash (cc:513-516), or the cell-grid key
+        keys[g] = S.key_off + (S.grid ? grid_key(cx, cy, cz, S.cell_lo, S.gdim, S.gmod)
+                                      : fastmod(hash, S.mod_m, S.table));
         wkeys[g] = work_key(E, S, s, cx, cy, cz);
     }
 }
@@ -302,7 +304,7 @@ __global__ void k_reorder(const EngineDesc *__restrict__ ed, const uint32_t *__r
 constexpr int KB = 4;
 
 template <bool SKIP_SELF, bool AUX, typename Body>
-__device__ __forceinline__ void walk(const SceneDesc &S, const float4 *__restrict__ xps,
+__device__ __forceinline__ void walk_hash(const SceneDesc &S, const float4 *__restrict__ xps,
                                      const float4 *__restrict__ aux,
                                      const uint32_t *__restrict__ start,
                                      const uint32_t *__restrict__ endt, uint32_t k,
@@ -360,6 +362,122 @@ __device__ __forceinline__ void walk(const SceneDesc &S, const float4 *__restric
     }
 }
 
+// The same walk under the cell-grid key.  The 27 cells, in the reference's
+// order, form 9 rows (dx, dy) of three cells dz = -1, 0, 1.  Where the three
+// cells of a row are consecutive buckets (everywhere but the grid's wrap
+// seam) their records are one contiguous run of sorted slots, walked as one;
+// a record belongs to the bucket its slot falls in and is tested against that
+// cell's float hash, exactly as in the hash walk.  At the seam the three
+// buckets are walked one after the other.  Either way `body` sees the same
+// records in the same order as under the reference key.  The bucket ranges of
+// the next row are fetched while the current row is walked.
+template <bool SKIP_SELF, bool AUX, typename Body>
+__device__ __forceinline__ void walk_grid(const SceneDesc &S, const float4 *__restrict__ xps,
+                                          const float4 *__restrict__ aux,
+                                          const uint32_t *__restrict__ start,
+                                          const uint32_t *__restrict__ endt, uint32_t k,
+                                          const float4 &pi, Body &&body)
+{
+    const int cx = cell_coord(pi.x, S.h), cy = cell_coord(pi.y, S.h), cz = cell_coord(pi.z, S.h);
+    const float thr = S.thr;
+    const uint32_t gy = S.gdim[1], gz = S.gdim[2], key_off = S.key_off;
+    const uint32_t ux = (uint32_t)cx - (uint32_t)S.cell_lo[0];
+    const uint32_t uy = (uint32_t)cy - (uint32_t)S.cell_lo[1];
+    const uint32_t uz = (uint32_t)cz - (uint32_t)S.cell_lo[2];
+    uint32_t wx[3], wy[3], wz[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        wx[d] = grid_wrap(ux + (uint32_t)(d - 1), S.gdim[0], S.gmod[0]);
+        wy[d] = grid_wrap(uy + (uint32_t)(d - 1), gy, S.gmod[1]);
+        wz[d] = grid_wrap(uz + (uint32_t)(d - 1), gz, S.gmod[2]);
+    }
+    const bool run = wz[1] == wz[0] + 1u && wz[2] == wz[1] + 1u;
+    const uint32_t h0 = hash_cell(cx, cy, cz);
+    uint32_t ns[3], ne[3];
+    auto fetch = [&](int r) {
+        const uint32_t base = key_off + (wx[r / 3] * gy + wy[r % 3]) * gz;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            ns[q] = start[base + wz[q]];
+            ne[q] = endt[base + wz[q]];  // meaningful only when ns[q] != SPH_START_EMPTY
+        }
+    };
+    fetch(0);
+#pragma unroll 1
+    for (int r = 0; r < 9; ++r) {
+        const uint32_t s0 = ns[0], s1 = ns[1], s2 = ns[2], e0 = ne[0], e1 = ne[1], e2 = ne[2];
+        if (r + 1 < 9) fetch(r + 1);
+        // float hashes of the row's cells (hash_cell is linear mod 2^32)
+        const uint32_t hr = h0 + (uint32_t)(r / 3 - 1) * 15823u + (uint32_t)(r % 3 - 1) * 9737333u;
+        const float f0 = (float)(hr - 440817757u), f1 = (float)hr, f2 = (float)(hr + 440817757u);
+        const bool n0 = s0 != SPH_START_EMPTY, n1 = s1 != SPH_START_EMPTY, n2 = s2 != SPH_START_EMPTY;
+        const int nseg = run ? 1 : 3;
+#pragma unroll 1
+        for (int q = 0; q < nseg; ++q) {
+            // slots [b, e); those below b1 belong to cell dz = -1 (hash g0),
+            // below b2 to dz = 0 (g1), the rest to dz = +1 (g2)
+            uint32_t b, e, b1, b2;
+            float g0, g1, g2;
+            if (run) {
+                b = n0 ? s0 : (n1 ? s1 : (n2 ? s2 : 0u));
+                e = n2 ? e2 : (n1 ? e1 : (n0 ? e0 : 0u));
+                b1 = n0 ? e0 : b;
+                b2 = n1 ? e1 : b1;
+                g0 = f0;
+                g1 = f1;
+                g2 = f2;
+            } else {
+                const bool nq = q == 0 ? n0 : (q == 1 ? n1 : n2);
+                b = nq ? (q == 0 ? s0 : (q == 1 ? s1 : s2)) : 0u;
+                e = nq ? (q == 0 ? e0 : (q == 1 ? e1 : e2)) : 0u;
+                b1 = b2 = e;
+                g0 = g1 = g2 = q == 0 ? f0 : (q == 1 ? f1 : f2);
+            }
+#pragma unroll 1
+            for (uint32_t m = b; m < e; m += KB) {
+                float4 rr[KB];
+#pragma unroll
+                for (int j = 0; j < KB; ++j) {
+                    rr[j] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(0x7fc00000));  // NaN: no match
+                    if (m + j < e) rr[j] = xps[m + j];
+                }
+                bool pass[KB];
+                float ox[KB], oy[KB], oz[KB], d2[KB];
+#pragma unroll
+                for (int j = 0; j < KB; ++j) {
+                    const uint32_t mj = m + j;
+                    const float hx = mj < b1 ? g0 : (mj < b2 ? g1 : g2);
+                    ox[j] = rr[j].x - pi.x;
+                    oy[j] = rr[j].y - pi.y;
+                    oz[j] = rr[j].z - pi.z;
+                    d2[j] = ox[j] * ox[j] + oy[j] * oy[j] + oz[j] * oz[j];  // glm::dot order
+                    pass[j] = rr[j].w == hx && !(SKIP_SELF && mj == k) && d2[j] < thr;
+                }
+                float4 a[KB];
+#pragma unroll
+                for (int j = 0; j < KB; ++j) {
+                    a[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    if (AUX && pass[j]) a[j] = aux[m + j];
+                }
+#pragma unroll
+                for (int j = 0; j < KB; ++j)
+                    if (pass[j]) body(m + j, ox[j], oy[j], oz[j], d2[j], a[j]);
+            }
+        }
+    }
+}
+
+template <bool GRID, bool SKIP_SELF, bool AUX, typename Body>
+__device__ __forceinline__ void walk(const SceneDesc &S, const float4 *__restrict__ xps,
+                                     const float4 *__restrict__ aux,
+                                     const uint32_t *__restrict__ start,
+                                     const uint32_t *__restrict__ endt, uint32_t k,
+                                     const float4 &pi, Body &&body)
+{
+    if constexpr (GRID) walk_grid<SKIP_SELF, AUX>(S, xps, aux, start, endt, k, pi, body);
+    else walk_hash<SKIP_SELF, AUX>(S, xps, aux, start, endt, k, pi, body);
+}
+
 // Gather kernels (density, pressure, viscosity), in result-identical
 // variants picked per workload by the autotuner (sph_engine.cpp):
 //
@@ -380,10 +498,12 @@ __device__ __forceinline__ void walk(const SceneDesc &S, const float4 *__restric
 // or from the list, never changes a result.
 constexpr uint32_t kListMax = 56;
 
-template <bool WORK>
+// GRID: work-item i handles sorted slot i (cells in grid order, already
+//   spatially coherent), with the XCD-contiguous workgroup numbering.
+template <bool WORK, bool GRID>
 __device__ __forceinline__ uint32_t gather_index()
 {
-    if (!WORK) return blockIdx.x * blockDim.x + threadIdx.x;
+    if (!WORK && !GRID) return blockIdx.x * blockDim.x + threadIdx.x;
     // workgroups are dealt round-robin to the 8 XCDs (block b -> XCD b % 8);
     // the grid is a multiple of 8
     const uint32_t G = gridDim.x, b = blockIdx.x;
@@ -395,7 +515,7 @@ __device__ __forceinline__ uint32_t gather_index()
 // (self excluded when SKIP_SELF), in walk order.  With LIST the candidate
 // tests and the bodies are split into the two phases described above; `lst`
 // is this work-item's column of the block's LDS list (stride blockDim.x).
-template <bool LIST, bool SKIP_SELF, bool AUX, typename Body>
+template <bool GRID, bool LIST, bool SKIP_SELF, bool AUX, typename Body>
 __device__ __forceinline__ void gather(const SceneDesc &S, const float4 *__restrict__ xps,
                                        const float4 *__restrict__ aux,
                                        const uint32_t *__restrict__ start,
@@ -403,12 +523,12 @@ __device__ __forceinline__ void gather(const SceneDesc &S, const float4 *__restr
                                        uint32_t k, const float4 &pi, Body &&body)
 {
     if (!LIST) {
-        walk<SKIP_SELF, AUX>(S, xps, aux, start, endt, k, pi, body);
+        walk<GRID, SKIP_SELF, AUX>(S, xps, aux, start, endt, k, pi, body);
         return;
     }
     const uint32_t stride = blockDim.x;
     uint32_t c = 0;
-    walk<SKIP_SELF, false>(S, xps, nullptr, start, endt, k, pi,
+    walk<GRID, SKIP_SELF, false>(S, xps, nullptr, start, endt, k, pi,
                            [&](uint32_t m, float, float, float, float, const float4 &) {
                                if (c < kListMax) lst[c * stride] = m;
                                ++c;
@@ -440,7 +560,7 @@ __device__ __forceinline__ void gather(const SceneDesc &S, const float4 *__restr
     }
     if (c > kListMax) {
         uint32_t skip = kListMax;
-        walk<SKIP_SELF, AUX>(S, xps, aux, start, endt, k, pi,
+        walk<GRID, SKIP_SELF, AUX>(S, xps, aux, start, endt, k, pi,
                              [&](uint32_t m, float ox, float oy, float oz, float d2,
                                  const float4 &a) {
                                  if (skip) --skip;
@@ -451,7 +571,7 @@ __device__ __forceinline__ void gather(const SceneDesc &S, const float4 *__restr
 
 #define SPH_GATHER_PROLOGUE()                                                       \
     extern __shared__ uint32_t lds_list[];                                          \
-    const uint32_t i = gather_index<WORK>();                                        \
+    const uint32_t i = gather_index<WORK, GRID>();                                  \
     if (i >= ed->n_total) return;                                                   \
     const uint32_t g = WORK ? work[i] : svals[i];                                   \
     const uint32_t k = WORK ? rank[g] : i;                                          \
@@ -487,7 +607,7 @@ __device__ __forceinline__ void gather_row(const SceneDesc &S, const float4 *__r
                                            uint32_t k, const float4 &pi, Body &&body)
 {
     if (c > kNbrMax) {
-        walk<true, true>(S, xps, aux, start, endt, k, pi, body);
+        walk<false, true, true>(S, xps, aux, start, endt, k, pi, body);
         return;
     }
     const uint4 *row4 = reinterpret_cast<const uint4 *>(row);
@@ -513,7 +633,7 @@ __device__ __forceinline__ void gather_row(const SceneDesc &S, const float4 *__r
     }
 }
 
-template <bool MULTI, bool WORK, bool LIST, bool NBR>
+template <bool MULTI, bool WORK, bool LIST, bool NBR, bool GRID>
 __global__ __launch_bounds__(256) void k_density(SPH_GATHER_PARAMS, float4 *__restrict__ dens)
 {
     SPH_GATHER_PROLOGUE();
@@ -521,7 +641,7 @@ __global__ __launch_bounds__(256) void k_density(SPH_GATHER_PARAMS, float4 *__re
     float rho = 0.0f, rhon = 0.0f;
     uint32_t *row = NBR ? nbr + (size_t)i * kNbrMax : nullptr;
     uint32_t c = 0;
-    gather<LIST, false, false>(S, xps, nullptr, start, endt, lst, k, pi,
+    gather<GRID, LIST, false, false>(S, xps, nullptr, start, endt, lst, k, pi,
                                [&](uint32_t m, float, float, float, float d2, const float4 &) {
         const float d = sqrtf(d2);
         const float v = h - d;
@@ -549,7 +669,7 @@ __global__ __launch_bounds__(256) void k_density_rows(SPH_GATHER_PARAMS, float4 
 {
     extern __shared__ uint32_t lds_rows[];  // [blockDim.x][kRowPitch], then counts[blockDim.x]
     const uint32_t B = blockDim.x, t = threadIdx.x;
-    const uint32_t i = gather_index<true>();
+    const uint32_t i = gather_index<true, false>();
     const uint32_t n = ed->n_total;
     uint32_t *my = lds_rows + t * kRowPitch;
     uint32_t *cnt = lds_rows + B * kRowPitch;
@@ -560,7 +680,7 @@ __global__ __launch_bounds__(256) void k_density_rows(SPH_GATHER_PARAMS, float4 
         const float4 pi = xps[k];
         const float h = S.h, w2c = S.w2c, w3c = S.w3c;
         float rho = 0.0f, rhon = 0.0f;
-        walk<false, false>(S, xps, nullptr, start, endt, k, pi,
+        walk<false, false, false>(S, xps, nullptr, start, endt, k, pi,
                            [&](uint32_t m, float, float, float, float d2, const float4 &) {
             const float d = sqrtf(d2);
             const float v = h - d;
@@ -586,7 +706,7 @@ __global__ __launch_bounds__(256) void k_density_rows(SPH_GATHER_PARAMS, float4 
     }
 }
 
-template <bool MULTI, bool WORK, bool LIST, bool NBR>
+template <bool MULTI, bool WORK, bool LIST, bool NBR, bool GRID>
 __global__ __launch_bounds__(256) void k_pressure(SPH_GATHER_PARAMS, const float4 *__restrict__ dens,
                                                    const float4 *__restrict__ vs,
                                                    float4 *__restrict__ vs2)
@@ -617,7 +737,7 @@ __global__ __launch_bounds__(256) void k_pressure(SPH_GATHER_PARAMS, const float
         fz += dirz * b * snp / dj.y;
     };
     if (NBR) gather_row(S, xps, dens, start, endt, nbr + (size_t)i * kNbrMax, ncnt[i], k, pi, body);
-    else gather<LIST, true, true>(S, xps, dens, start, endt, lst, k, pi, body);
+    else gather<GRID, LIST, true, true>(S, xps, dens, start, endt, lst, k, pi, body);
     float4 v = vs[k];
     const float dt = S.dt, rho = di.x;
     v.x = v.x + (fx / rho) * dt;  // cc:421
@@ -626,7 +746,7 @@ __global__ __launch_bounds__(256) void k_pressure(SPH_GATHER_PARAMS, const float
     vs2[k] = v;
 }
 
-template <bool MULTI, bool WORK, bool LIST, bool NBR>
+template <bool MULTI, bool WORK, bool LIST, bool NBR, bool GRID>
 __global__ __launch_bounds__(256) void k_viscosity_integrate(SPH_GATHER_PARAMS,
                                                               const float4 *__restrict__ vs2,
                                                               float4 *__restrict__ pos,
@@ -645,7 +765,7 @@ __global__ __launch_bounds__(256) void k_viscosity_integrate(SPH_GATHER_PARAMS,
         fz += (vj.z - vi.z) * w;
     };
     if (NBR) gather_row(S, xps, vs2, start, endt, nbr + (size_t)i * kNbrMax, ncnt[i], k, pi, body);
-    else gather<LIST, true, true>(S, xps, vs2, start, endt, lst, k, pi, body);
+    else gather<GRID, LIST, true, true>(S, xps, vs2, start, endt, lst, k, pi, body);
     const float dt = S.dt, mu = S.mu;
     float4 v = vi;
     v.x = vi.x + fx * mu * dt;  // cc:463
@@ -691,7 +811,9 @@ __device__ __forceinline__ void walk_prefilter(const SceneDesc &S, const float4 
         for (int dy = -1; dy <= 1; ++dy)
             for (int dz = -1; dz <= 1; ++dz) {
                 const uint32_t hash = hash_cell(cx + dx, cy + dy, cz + dz);
-                const uint32_t key = S.key_off + fastmod(hash, S.mod_m, S.table);
+                const uint32_t key =
+                    S.key_off + (S.grid ? grid_key(cx + dx, cy + dy, cz + dz, S.cell_lo, S.gdim, S.gmod)
+                                        : fastmod(hash, S.mod_m, S.table));
                 const uint32_t b = start[key];
                 if (b == SPH_START_EMPTY) continue;
                 const uint32_t e = endt[key];
@@ -831,7 +953,7 @@ inline uint32_t gather_grid(uint32_t cap, int block)
     return (blocks + 7u) & ~7u;
 }
 
-// Launches kernel template KER<MULTI, WORK, LIST> chosen at run time.
+// Launches kernel template KER<MULTI, WORK, LIST, NBR, GRID> chosen at run time.
 #define SPH_GATHER_LAUNCH(KER, var, blk, st, ...)                                             \
     do {                                                                                      \
         const GatherVariant v_ = (var);                                                       \
@@ -839,18 +961,23 @@ inline uint32_t gather_grid(uint32_t cap, int block)
         const uint32_t g_ = gather_grid(s.capacity, b_);                                      \
         const size_t lds_ = v_.list ? sizeof(uint32_t) * kListMax * (size_t)b_ : 0;           \
         const bool m_ = s.multi;                                                              \
-        if (c.nbr && v_.rows) {                                                               \
-            if (m_) KER<true, true, false, true><<<g_, b_, 0, st>>>(__VA_ARGS__);             \
-            else KER<false, true, false, true><<<g_, b_, 0, st>>>(__VA_ARGS__);               \
+        if (s.grid) {                                                                         \
+            if (m_ && v_.list) KER<true, false, true, false, true><<<g_, b_, lds_, st>>>(__VA_ARGS__);   \
+            else if (m_) KER<true, false, false, false, true><<<g_, b_, lds_, st>>>(__VA_ARGS__);        \
+            else if (v_.list) KER<false, false, true, false, true><<<g_, b_, lds_, st>>>(__VA_ARGS__);   \
+            else KER<false, false, false, false, true><<<g_, b_, lds_, st>>>(__VA_ARGS__);               \
+        } else if (c.nbr && v_.rows) {                                                        \
+            if (m_) KER<true, true, false, true, false><<<g_, b_, 0, st>>>(__VA_ARGS__);      \
+            else KER<false, true, false, true, false><<<g_, b_, 0, st>>>(__VA_ARGS__);        \
         } else if (m_ && v_.work && v_.list)                                                  \
-            KER<true, true, true, false><<<g_, b_, lds_, st>>>(__VA_ARGS__);                  \
-        else if (m_ && v_.work) KER<true, true, false, false><<<g_, b_, lds_, st>>>(__VA_ARGS__);   \
-        else if (m_ && v_.list) KER<true, false, true, false><<<g_, b_, lds_, st>>>(__VA_ARGS__);   \
-        else if (m_) KER<true, false, false, false><<<g_, b_, lds_, st>>>(__VA_ARGS__);             \
-        else if (v_.work && v_.list) KER<false, true, true, false><<<g_, b_, lds_, st>>>(__VA_ARGS__); \
-        else if (v_.work) KER<false, true, false, false><<<g_, b_, lds_, st>>>(__VA_ARGS__);        \
-        else if (v_.list) KER<false, false, true, false><<<g_, b_, lds_, st>>>(__VA_ARGS__);        \
-        else KER<false, false, false, false><<<g_, b_, lds_, st>>>(__VA_ARGS__);                    \
+            KER<true, true, true, false, false><<<g_, b_, lds_, st>>>(__VA_ARGS__);           \
+        else if (m_ && v_.work) KER<true, true, false, false, false><<<g_, b_, lds_, st>>>(__VA_ARGS__);   \
+        else if (m_ && v_.list) KER<true, false, true, false, false><<<g_, b_, lds_, st>>>(__VA_ARGS__);   \
+        else if (m_) KER<true, false, false, false, false><<<g_, b_, lds_, st>>>(__VA_ARGS__);             \
+        else if (v_.work && v_.list) KER<false, true, true, false, false><<<g_, b_, lds_, st>>>(__VA_ARGS__); \
+        else if (v_.work) KER<false, true, false, false, false><<<g_, b_, lds_, st>>>(__VA_ARGS__);        \
+        else if (v_.list) KER<false, false, true, false, false><<<g_, b_, lds_, st>>>(__VA_ARGS__);        \
+        else KER<false, false, false, false, false><<<g_, b_, lds_, st>>>(__VA_ARGS__);                    \
     } while (0)
 
 #define SPH_GATHER_ARGS                                                                \
@@ -858,7 +985,7 @@ inline uint32_t gather_grid(uint32_t cap, int block)
 
 hipError_t launch_density(const DevState &s, const LaunchCfg &c, int buf, hipStream_t st)
 {
-    if (c.nbr) {
+    if (c.nbr && !s.grid) {
         const int b = c.density.block;
         const size_t lds = sizeof(uint32_t) * ((size_t)b * kRowPitch + b);
         if (s.multi)

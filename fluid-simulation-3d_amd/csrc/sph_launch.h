@@ -15,6 +15,7 @@ struct DevState {
     const SceneDesc *sd;
     const uint32_t *scene_of;  // [n] scene of each slot (unused when single)
     bool multi;                // more than one scene
+    bool grid;                 // cell-grid bucket key (SceneDesc::grid of every scene)
     // particle state, id order
     float4 *pos;  // x,y,z, w = 0.34 after an update (OutPositions layout, cc:107)
     float4 *vel;  // x,y,z, w unused
@@ -61,6 +62,32 @@ struct LaunchCfg {
     int block_stream = 256;  // threads per block of streaming kernels
     bool needs_work() const { return nbr || density.work || pressure.work || viscosity.work; }
 };
+
+// Defaults per bucket key.  Under the cell-grid key the sorted order is
+// already spatially coherent: the kernels run in it (no Morton work order,
+// no neighbour rows).
+inline LaunchCfg sanitize_launch_cfg(LaunchCfg c, bool grid)
+{
+    if (grid) {
+        for (GatherVariant *v : {&c.density, &c.pressure, &c.viscosity}) {
+            v->work = false;
+            v->rows = false;
+        }
+        c.nbr = false;
+    }
+    return c;
+}
+
+inline LaunchCfg default_launch_cfg(bool grid)
+{
+    LaunchCfg c;
+    if (grid) {
+        c.density = GatherVariant{false, false, 256};
+        c.pressure = GatherVariant{false, true, 128};
+        c.viscosity = GatherVariant{false, false, 256};
+    }
+    return sanitize_launch_cfg(c, grid);
+}
 
 // Stage A + B1: predict, cell, hash, key; vals = slot index.
 hipError_t launch_predict(const DevState &s, const LaunchCfg &c, hipStream_t st);

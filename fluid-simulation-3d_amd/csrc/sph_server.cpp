@@ -122,7 +122,7 @@ int ensure_capacity(sph_server_t *s, uint64_t need_n, uint64_t need_t, std::stri
     }
     sph::Engine *ne = nullptr;
     int rc = sph::Engine::create({}, (uint32_t)cap, (uint32_t)tcap, s->cfg.max_running, &ne, err,
-                                 true);
+                                 true, true);
     if (rc) return rc;
     if (e) {
         // carry the running scenes over through host memory (rare: growth only)
@@ -220,9 +220,10 @@ void sph_server_default_config(sph_server_config *c)
     c->updates_per_tick = 1;
 }
 
-uint64_t sph_server_request_bytes(uint32_t n, uint32_t table_size)
+uint64_t sph_server_request_bytes(const sph_params *p, uint32_t n, uint32_t table_size)
 {
-    return request_bytes(n, table_size ? table_size : n);
+    if (!p || sph::check_params(*p)) return 0;
+    return request_bytes(n, table_size ? table_size : sph::grid_table_size(*p, n));
 }
 
 int sph_server_create(const sph_server_config *cfg, sph_server *out)
@@ -281,7 +282,7 @@ int sph_server_submit(sph_server s, const sph_params *p, uint32_t n, uint32_t ta
     r.id = s->next_id++;
     r.params = *p;
     r.n = n;
-    r.table = table_size ? table_size : n;
+    r.table = table_size ? table_size : sph::grid_table_size(*p, n);
     r.bytes = request_bytes(r.n, r.table);
     if (r.bytes > s->cfg.hbm_budget_bytes) {
         ++s->rejected;

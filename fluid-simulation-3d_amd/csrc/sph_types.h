@@ -1,6 +1,22 @@
 // sph_types.h -- data structures shared by host code and the gfx950 kernels.
 //
 // One "engine" holds S >= 1 independent scenes packed into flat SoA arrays.
+//
+// Bucket keys.  The reference buckets particles by hash % T (cc:513-516) and
+// walks, for each of the 27 neighbour cells, that cell's bucket, keeping the
+// records whose float-rounded hash equals the cell's.  A particle within h of
+// particle i always lies in one of i's 27 cells, and no two of those cells
+// have float-equal hashes: two of them differ by (dx, dy, dz) in [-2, 2]^3,
+// so their hashes differ by dx*15823 + dy*9737333 + dz*440817757 mod 2^32,
+// which is at least 15823 away from 0 mod 2^32, while u32 values that round
+// to the same float are less than 256 apart.  Each sum of the reference is
+// therefore: for each offset in the reference order, the particles of that
+// cell in ascending index (the stable sort).  That holds for ANY key that is
+// a function of the cell.  The engine
+// can therefore use either the reference key or the cell-grid key
+// ((wx * Gy + wy) * Gz + wz with w = (c - cell_lo) mod G per axis), under
+// which the three cells dz = -1, 0, 1 of a neighbour row are consecutive
+// buckets, i.e. one contiguous run of sorted slots.  Results are identical.
 // Scene s owns particle slots [p_off, p_off + n) -- the same range in id order
 // and in sorted order, because every key of scene s lies in
 // [key_off, key_off + table) and the key ranges of scenes are disjoint and
@@ -34,8 +50,11 @@ struct SceneDesc {
     float half[3];        // BoundScale * 0.5f (cc:88)
     float damp_neg;       // -1 * dampFactor (cc:94)
     float dt;             // substep dt of the current Update
-    int32_t cell_lo[3];   // origin of the scene's cell grid for the work-order key
-    uint32_t pad[1];
+    int32_t cell_lo[3];   // origin of the scene's cell grid (grid key, work-order key)
+    uint32_t grid;        // bucket key: 1 = cell-grid key, 0 = hash % table (cc:513-516)
+    uint32_t gdim[3];     // grid key: period of the cell grid per axis, product <= table
+    uint32_t pad;
+    uint64_t gmod[3];     // grid key: remainder constants of gdim
 };
 
 // Whole-engine counters, resident in device memory so that a captured graph

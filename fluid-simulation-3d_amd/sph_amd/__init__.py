@@ -29,6 +29,10 @@ SPH_ERR_BUDGET = -4
 SPH_ERR_STATE = -5
 SPH_ERR_NOT_FOUND = -6
 SPH_ERR_NONFINITE = -7
+
+# bucket keys of sph_create's table_size (sph_amd.h)
+TABLE_GRID = 0
+TABLE_REFERENCE = 0xFFFFFFFF
 START_EMPTY = 0x7FFFFFFF  # INT_MAX, physicsWorld.cc:481
 
 INIT_GRID_ARRANGEMENT = 0
@@ -143,7 +147,8 @@ def lib():
         "sph_debug_neighbours": (i32, [H, p, p, u64, p]),
         "sph_generate": (i32, [i32, u64, u32, p, f, f, f, p, p]),
         "sph_server_default_config": (None, [ctypes.POINTER(ServerConfig)]),
-        "sph_server_request_bytes": (u64, [u32, u32]),
+        "sph_server_request_bytes": (u64, [ctypes.POINTER(Params), u32, u32]),
+        "sph_get_key": (i32, [H, p, p, p]),
         "sph_server_create": (i32, [ctypes.POINTER(ServerConfig), ctypes.POINTER(H)]),
         "sph_server_destroy": (i32, [H]),
         "sph_server_submit": (i32, [H, pp, u32, u32, p, p, u32, p]),
@@ -253,7 +258,9 @@ def _variants_array(d):
 class Sim:
     """One scene on the GPU: the FluidSimulation of physicsWorld.h:28-155."""
 
-    def __init__(self, params, n, table_size=0):
+    def __init__(self, params, n, table_size=TABLE_GRID):
+        """table_size: TABLE_GRID (cell-grid key), TABLE_REFERENCE (hash % n) or
+        T (hash % T); results are identical under every key."""
         self._h = ctypes.c_void_p()
         self.n = int(n)
         _check(lib().sph_create(ctypes.byref(params), self.n, table_size, ctypes.byref(self._h)))
@@ -354,6 +361,14 @@ class Sim:
         c = np.zeros(1, np.uint64)
         _check(lib().sph_count_nonfinite(self._h, _ptr(c)))
         return int(c[0])
+
+    def key(self):
+        """(grid, gdim, lo): the bucket key in use (grid False: hash % table)."""
+        g = np.zeros(1, np.int32)
+        gd = np.zeros(3, np.uint32)
+        lo = np.zeros(3, np.int32)
+        _check(lib().sph_get_key(self._h, _ptr(g), _ptr(gd), _ptr(lo)))
+        return bool(g[0]), [int(x) for x in gd], [int(x) for x in lo]
 
     def debug_spatial(self):
         n, T = self.n, self.table
@@ -462,8 +477,9 @@ class Batch:
         _check(lib().sph_batch_set_variants(self._h, _ptr(_variants_array(d))))
 
 
-def request_bytes(n, table=0):
-    return int(lib().sph_server_request_bytes(int(n), int(table)))
+def request_bytes(params, n, table=0):
+    """Device bytes a server request holds while running (0 for invalid settings)."""
+    return int(lib().sph_server_request_bytes(ctypes.byref(params), int(n), int(table)))
 
 
 class Server:

@@ -85,9 +85,21 @@ int sph_copy_calibration(uint64_t bytes, int reps, double *ms_per_copy);
 
 /* ---- single scene: FluidSimulation (physicsWorld.h:28-155) ---- */
 
+/* Bucket keys of the spatial lookup (UpdateSpatialLookup, cc:466-498).  The
+ * results of every update are bit-identical under either key; only the lookup
+ * tables (sph_debug_spatial) and the speed differ.
+ *   SPH_TABLE_GRID (0): the cell-grid key -- buckets are the cells of a grid
+ *     over the box (+4 cells per side), z fastest, so a neighbour row of three
+ *     cells is one contiguous run of sorted slots; cells outside the grid
+ *     wrap around it.  The bucket count is chosen from bounds and h.
+ *   SPH_TABLE_REFERENCE: the reference's key hash % n (GetKeyFromHash with
+ *     tableSize = numParticles, cc:479, 513-516).
+ *   any other T: the reference's key with T buckets, hash % T. */
+#define SPH_TABLE_GRID 0u
+#define SPH_TABLE_REFERENCE 0xffffffffu
+
 /* FluidSimulation::getInstance + InitializeData's sizing (h:32,36; cc:112-137):
- * a scene of n particles whose spatial hash has table_size buckets
- * (0 = n, the reference's choice: GetKeyFromHash(hash, numParticles), cc:479).
+ * a scene of n particles; table_size selects the bucket key (above).
  * All state starts at zero. */
 int sph_create(const sph_params *p, uint32_t n, uint32_t table_size, sph_handle *out);
 int sph_destroy(sph_handle h);
@@ -95,6 +107,11 @@ int sph_destroy(sph_handle h);
 int sph_set_params(sph_handle h, const sph_params *p);
 int sph_get_params(sph_handle h, sph_params *p);
 int sph_get_size(sph_handle h, uint32_t *n, uint32_t *table_size);
+/* Bucket key in use: *grid = 1 for the cell-grid key, with its periods
+ * gdim[3] and origin cell lo[3] (key = (wx*Gy + wy)*Gz + wz, w = (u32(cell) -
+ * u32(lo)) mod G per axis); *grid = 0 for hash % table_size.  Any output may
+ * be NULL.  No reference counterpart (parity tests). */
+int sph_get_key(sph_handle h, int32_t *grid, uint32_t gdim[3], int32_t lo[3]);
 /* Writes the positions vector (h:79) and the velocities; either may be NULL
  * (left unchanged).  Host float4 arrays of n entries. */
 int sph_upload(sph_handle h, const float *pos4, const float *vel4);
@@ -151,8 +168,8 @@ int sph_count_nonfinite(sph_handle h, uint64_t *count);
  * UpdateSpatialLookup (cc:466-498): per particle hash (HashCell) and key
  * (GetKeyFromHash); per sorted slot key and particle index; per bucket the
  * first sorted slot (startIndices, INT_MAX when empty) and one past the last.
- * Any pointer may be NULL.  hash_id/key_id/sorted_* hold n entries,
- * start/end table_size entries. */
+ * Keys are the bucket key in use (sph_get_key).  Any pointer may be NULL.
+ * hash_id/key_id/sorted_* hold n entries, start/end table_size entries. */
 int sph_debug_spatial(sph_handle h, uint32_t *hash_id, uint32_t *key_id,
                       uint32_t *sorted_key, uint32_t *sorted_idx, uint32_t *start,
                       uint32_t *end);
@@ -179,8 +196,9 @@ uint32_t sph_bucket_of(uint32_t hash, uint32_t table_size);
  * are offset per scene so scenes never interact), and the update is replayed
  * from a captured HIP graph.  Each scene's results are bit-identical to the
  * same scene run alone through sph_create/sph_update.
- * counts[s] particles and table_sizes[s] buckets (table_sizes NULL or an
- * entry 0 = counts[s]); params[s] settings. */
+ * counts[s] particles and params[s] settings.  table_sizes NULL: every scene
+ * uses the cell-grid key; else the reference key with table_sizes[s] buckets
+ * (an entry 0 or SPH_TABLE_REFERENCE = counts[s]). */
 typedef struct sph_batch_t *sph_batch;
 int sph_batch_create(uint32_t n_scenes, const uint32_t *counts, const uint32_t *table_sizes,
                      const sph_params *params, sph_batch *out);
@@ -201,7 +219,7 @@ int sph_batch_set_variants(sph_batch b, const int32_t v[13]);
 
 /* ---- serving: continuous batching of bake requests ----
  * No reference counterpart.  A request is a scene (settings, particle count,
- * table size, initial state) to advance by `steps` steps of `substeps`
+ * bucket count, initial state) to advance by `steps` steps of `substeps`
  * Update(dt / substeps) calls.  Requests wait in arrival order and are
  * admitted while one of max_running slots is free and the device bytes of
  * the running requests (sph_server_request_bytes each) stay within
@@ -210,8 +228,10 @@ int sph_batch_set_variants(sph_batch b, const int32_t v[13]);
  * every running request by updates_per_tick Update calls -- one launch per
  * kernel for all of them -- and retires the finished ones, whose results are
  * kept in host memory until fetched.  Each result is bit-identical to the
- * same scene run alone.  Calls on one server must be serialised by the
- * caller. */
+ * same scene run alone.  Every request uses the cell-grid key; its
+ * table_size is the bucket count (0: sized from bounds and h, as
+ * SPH_TABLE_GRID; a smaller count wraps the grid, results are unchanged).
+ * Calls on one server must be serialised by the caller. */
 #define SPH_REQ_QUEUED 0
 #define SPH_REQ_RUNNING 1
 #define SPH_REQ_DONE 2
@@ -241,7 +261,8 @@ typedef struct sph_server_stats {
 
 typedef struct sph_server_t *sph_server;
 void sph_server_default_config(sph_server_config *c);
-uint64_t sph_server_request_bytes(uint32_t n, uint32_t table_size);
+/* Device bytes a request holds while running (its HBM-budget charge). */
+uint64_t sph_server_request_bytes(const sph_params *p, uint32_t n, uint32_t table_size);
 int sph_server_create(const sph_server_config *cfg, sph_server *out);
 int sph_server_destroy(sph_server s);
 int sph_server_submit(sph_server s, const sph_params *p, uint32_t n, uint32_t table_size,

@@ -126,6 +126,30 @@ uint32_t orc_hash_cell(const float cell[3])
 
 uint32_t orc_key(uint32_t hash, uint32_t table_size) { return hash % table_size; } /* cc:513-516 */
 
+/* Test hook (no reference counterpart): the cell-grid bucket key. */
+static int g_grid_key = 0;
+static uint32_t g_gdim[3];
+static int32_t g_glo[3];
+
+void orc_set_grid_key(const uint32_t *gdim, const int32_t *lo)
+{
+    g_grid_key = gdim != NULL && lo != NULL;
+    for (int c = 0; c < 3; ++c) {
+        g_gdim[c] = g_grid_key ? gdim[c] : 0;
+        g_glo[c] = g_grid_key ? lo[c] : 0;
+    }
+}
+
+/* Bucket of a cell: the reference's hash % T, or (hook set) the cell-grid
+ * key (wx * Gy + wy) * Gz + wz with w = (u32(cell) - u32(lo)) mod G. */
+static inline uint32_t cell_key(const float cell[3], uint32_t hash, uint32_t T)
+{
+    if (!g_grid_key) return orc_key(hash, T);
+    uint32_t w[3];
+    for (int c = 0; c < 3; ++c) w[c] = (f2u_wrap(cell[c]) - (uint32_t)g_glo[c]) % g_gdim[c];
+    return (w[0] * g_gdim[1] + w[1]) * g_gdim[2] + w[2];
+}
+
 /* ------------------------------------------------------------------ */
 /* stages                                                              */
 /* ------------------------------------------------------------------ */
@@ -150,6 +174,8 @@ int orc_stage_spatial(uint32_t n, uint32_t T, float h, const float *xp4, uint32_
                       uint32_t *start)
 {
     if (n == 0 || T == 0 || n > (1u << 24) || T > (1u << 24) || !start) return -1;
+    if (g_grid_key && (uint64_t)g_gdim[0] * g_gdim[1] * g_gdim[2] > T) return -1;
+    if (g_grid_key && (g_gdim[0] == 0 || g_gdim[1] == 0 || g_gdim[2] == 0)) return -1;
     uint32_t *hid = hash_id ? hash_id : (uint32_t *)malloc(sizeof(uint32_t) * n);
     uint32_t *kid = key_id ? key_id : (uint32_t *)malloc(sizeof(uint32_t) * n);
     uint32_t *sk = sorted_key ? sorted_key : (uint32_t *)malloc(sizeof(uint32_t) * n);
@@ -165,7 +191,7 @@ int orc_stage_spatial(uint32_t n, uint32_t T, float h, const float *xp4, uint32_
         orc_cell_coord(&xp4[4 * i], h, cell);
         uint32_t hash = orc_hash_cell(cell);
         hid[i] = hash;
-        kid[i] = orc_key(hash, T);
+        kid[i] = cell_key(cell, hash, T);
     }
     for (uint32_t k = 0; k < T; ++k) start[k] = ORC_START_EMPTY;
 
@@ -206,7 +232,7 @@ out:
             float c_[3] = {cell_[0] + OFFS[oi_][0], cell_[1] + OFFS[oi_][1],              \
                            cell_[2] + OFFS[oi_][2]};                                      \
             uint32_t hash_ = orc_hash_cell(c_);                                           \
-            uint32_t key_ = orc_key(hash_, T);                                            \
+            uint32_t key_ = cell_key(c_, hash_, T);                                       \
             uint32_t cur_ = start[key_];                                                  \
             while (cur_ < n) {                                                            \
                 uint32_t ridx_ = sorted_idx[cur_];                                        \

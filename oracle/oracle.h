@@ -77,6 +77,13 @@ void orc_cell_coord(const float xp[3], float h, float cell[3]);
 uint32_t orc_hash_cell(const float cell[3]);
 uint32_t orc_key(uint32_t hash, uint32_t table_size);
 
+/* Test hook, no reference counterpart.  Switches the bucket key of
+ * orc_stage_spatial and of every walk from hash % T to the cell-grid key
+ * (wx * Gy + wy) * Gz + wz, w = (u32(cell) - u32(lo)) mod G per axis
+ * (T must be >= Gx * Gy * Gz); NULL arguments restore the reference key.
+ * The sums of a step must not depend on which key is used.  Not thread-safe. */
+void orc_set_grid_key(const uint32_t *gdim, const int32_t *lo);
+
 /* ---- stages of Update (physicsWorld.cc:39-111) over float4 (x,y,z,w) arrays ---- */
 /* Stage A, cc:42-48: vel += g*dt; xp = pos + vel*lookahead.  xp4[.w] = 0. */
 void orc_stage_predict(uint32_t n, const orc_params *p, float dt,

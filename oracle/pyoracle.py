@@ -57,6 +57,7 @@ def lib():
             "orc_cell_coord": (None, [p, f, p]),
             "orc_hash_cell": (u32, [p]),
             "orc_key": (u32, [u32, u32]),
+            "orc_set_grid_key": (None, [p, p]),
             "orc_stage_predict": (None, [u32, pp, f, p, p, p]),
             "orc_stage_spatial": (i32, [u32, u32, f, p, p, p, p, p, p]),
             "orc_stage_density": (None, [u32, u32, pp, p, p, p, p, p, p]),
@@ -129,6 +130,32 @@ def hash_cell(cell):
 
 def key(hash_, T):
     return lib().orc_key(hash_, T)
+
+
+def set_grid_key(gdim=None, lo=None):
+    """Test hook: bucket by the cell-grid key (gdim, lo) instead of hash % T;
+    no arguments restore the reference key."""
+    if gdim is None:
+        lib().orc_set_grid_key(None, None)
+        return
+    g = np.ascontiguousarray(gdim, dtype=np.uint32)
+    o = np.ascontiguousarray(lo, dtype=np.int32)
+    lib().orc_set_grid_key(_ptr(g), _ptr(o))
+
+
+class grid_key:
+    """Context manager form of set_grid_key."""
+
+    def __init__(self, gdim, lo):
+        self.gdim, self.lo = gdim, lo
+
+    def __enter__(self):
+        set_grid_key(self.gdim, self.lo)
+        return self
+
+    def __exit__(self, *exc):
+        set_grid_key()
+        return False
 
 
 class Spatial:

@@ -60,7 +60,7 @@ def test_requests_alone_equal_mixed():
 def test_budget_admission_and_refusal():
     a = _scene(5000, 21)
     b = _scene(7000, 22)
-    budget = S.request_bytes(5000) + S.request_bytes(7000) - 1
+    budget = S.request_bytes(a[0], 5000) + S.request_bytes(b[0], 7000) - 1
     with S.Server(hbm_budget_bytes=budget, max_running=8, initial_capacity=1 << 16) as srv:
         ia = srv.submit(a[0], a[1], a[2], 2)
         ib = srv.submit(b[0], b[1], b[2], 2)
