@@ -371,12 +371,78 @@ __device__ __forceinline__ void walk_hash(const SceneDesc &S, const float4 *__re
 // buckets are walked one after the other.  Either way `body` sees the same
 // records in the same order as under the reference key.  The bucket ranges of
 // the next row are fetched while the current row is walked.
+//
+// Row staging (stage != nullptr): the work-items of a wave hold consecutive
+// sorted slots -- a z-segment of one grid column -- so their runs in a row
+// overlap inside one short slot range [lo, hi) (lo = the first active lane's
+// run start, hi = the last one's end).  When every active lane's run lies in
+// that range and it fits kStageCap records, the wave copies it into its LDS
+// buffer with coalesced loads and the lanes walk their runs from LDS.  The
+// check is exact per lane, so the range assumption never affects a result;
+// rows that fail it are walked from global memory.
+constexpr uint32_t kStageCap = 120;            // staged records per wave and row
+constexpr uint32_t kStageBuf = kStageCap + KB;  // + padding: a batch may read KB-1 past the end
+
+// Slots [b, e) of a grid row: the buckets of the row's cells dz = -1, 0, 1
+// (float hashes g0, g1, g2), in that order.  A record passes when d2 < thr
+// and its float hash is one of g0, g1, g2.  That equals the reference's test
+// (hash of the record's own bucket cell): a record with d2 < thr lies within
+// h, so its cell is one of the 27 around the particle, whose float hashes are
+// pairwise distinct; matching g_q therefore means it is a particle of cell q,
+// which sits in bucket q.  The tests are combined without short-circuit
+// branches.  STAGED: the records are read from the wave's LDS copy
+// src[slot - base] of a range containing [b, e) (the padded buffer makes the
+// reads past e harmless; they never pass); else from xps in global memory.
+template <bool STAGED, bool SKIP_SELF, bool AUX, typename Body>
+__device__ __forceinline__ void walk_run(const float4 *src, uint32_t base, const float4 *__restrict__ xps,
+                                         const float4 *__restrict__ aux, uint32_t k,
+                                         const float4 &pi, float thr, uint32_t b, uint32_t e,
+                                         float g0, float g1, float g2, Body &body)
+{
+#pragma unroll 1
+    for (uint32_t m = b; m < e; m += KB) {
+        float4 rr[KB];
+#pragma unroll
+        for (int j = 0; j < KB; ++j) {
+            if (STAGED) {
+                rr[j] = src[m + j - base];
+            } else {
+                rr[j] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(0x7fc00000));  // NaN: no match
+                if (m + j < e) rr[j] = xps[m + j];
+            }
+        }
+        bool pass[KB];
+        float ox[KB], oy[KB], oz[KB], d2[KB];
+#pragma unroll
+        for (int j = 0; j < KB; ++j) {
+            const uint32_t mj = m + j;
+            ox[j] = rr[j].x - pi.x;
+            oy[j] = rr[j].y - pi.y;
+            oz[j] = rr[j].z - pi.z;
+            d2[j] = ox[j] * ox[j] + oy[j] * oy[j] + oz[j] * oz[j];  // glm::dot order
+            const float w = rr[j].w;
+            pass[j] = ((int)(!STAGED || mj < e) & (int)(d2[j] < thr) &
+                       ((int)(w == g0) | (int)(w == g1) | (int)(w == g2)) &
+                       (int)(!SKIP_SELF || mj != k)) != 0;
+        }
+        float4 a[KB];
+#pragma unroll
+        for (int j = 0; j < KB; ++j) {
+            a[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (AUX && pass[j]) a[j] = aux[m + j];
+        }
+#pragma unroll
+        for (int j = 0; j < KB; ++j)
+            if (pass[j]) body(m + j, ox[j], oy[j], oz[j], d2[j], a[j]);
+    }
+}
+
 template <bool SKIP_SELF, bool AUX, typename Body>
 __device__ __forceinline__ void walk_grid(const SceneDesc &S, const float4 *__restrict__ xps,
                                           const float4 *__restrict__ aux,
                                           const uint32_t *__restrict__ start,
                                           const uint32_t *__restrict__ endt, uint32_t k,
-                                          const float4 &pi, Body &&body)
+                                          const float4 &pi, float4 *stage, Body &&body)
 {
     const int cx = cell_coord(pi.x, S.h), cy = cell_coord(pi.y, S.h), cz = cell_coord(pi.z, S.h);
     const float thr = S.thr;
@@ -411,57 +477,44 @@ __device__ __forceinline__ void walk_grid(const SceneDesc &S, const float4 *__re
         const uint32_t hr = h0 + (uint32_t)(r / 3 - 1) * 15823u + (uint32_t)(r % 3 - 1) * 9737333u;
         const float f0 = (float)(hr - 440817757u), f1 = (float)hr, f2 = (float)(hr + 440817757u);
         const bool n0 = s0 != SPH_START_EMPTY, n1 = s1 != SPH_START_EMPTY, n2 = s2 != SPH_START_EMPTY;
-        const int nseg = run ? 1 : 3;
-#pragma unroll 1
-        for (int q = 0; q < nseg; ++q) {
-            // slots [b, e); those below b1 belong to cell dz = -1 (hash g0),
-            // below b2 to dz = 0 (g1), the rest to dz = +1 (g2)
-            uint32_t b, e, b1, b2;
-            float g0, g1, g2;
-            if (run) {
-                b = n0 ? s0 : (n1 ? s1 : (n2 ? s2 : 0u));
-                e = n2 ? e2 : (n1 ? e1 : (n0 ? e0 : 0u));
-                b1 = n0 ? e0 : b;
-                b2 = n1 ? e1 : b1;
-                g0 = f0;
-                g1 = f1;
-                g2 = f2;
-            } else {
-                const bool nq = q == 0 ? n0 : (q == 1 ? n1 : n2);
-                b = nq ? (q == 0 ? s0 : (q == 1 ? s1 : s2)) : 0u;
-                e = nq ? (q == 0 ? e0 : (q == 1 ? e1 : e2)) : 0u;
-                b1 = b2 = e;
-                g0 = g1 = g2 = q == 0 ? f0 : (q == 1 ? f1 : f2);
+        if (run) {
+            // the row's merged run
+            const uint32_t rb = n0 ? s0 : (n1 ? s1 : (n2 ? s2 : 0u));
+            const uint32_t re = n2 ? e2 : (n1 ? e1 : (n0 ? e0 : 0u));
+            bool staged = false;
+            uint32_t lo = 0;
+            if (stage) {
+                // staging decision, uniform over the wave's active lanes
+                const uint64_t act = __ballot(1);
+                const int first = __builtin_ctzll(act), last = 63 - __builtin_clzll(act);
+                lo = (uint32_t)__builtin_amdgcn_readlane((int)rb, first);
+                const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)re, last);
+                const bool inside = rb == re || (rb >= lo && re <= hi);
+                if (__ballot(inside) == act && hi >= lo && hi - lo <= kStageCap) {
+                    const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+                        (uint32_t)(act >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0u));
+                    const uint32_t cnt = (uint32_t)__builtin_popcountll(act);
+                    __builtin_amdgcn_wave_barrier();  // the previous row's LDS reads are issued
+                    for (uint32_t t = rank; t < hi - lo; t += cnt) stage[t] = xps[lo + t];
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    staged = true;
+                }
             }
+            if (staged)
+                walk_run<true, SKIP_SELF, AUX>(stage, lo, xps, aux, k, pi, thr, rb, re, f0, f1, f2, body);
+            else
+                walk_run<false, SKIP_SELF, AUX>(xps, 0, xps, aux, k, pi, thr, rb, re, f0, f1, f2, body);
+        } else {
+            // wrap seam: the three buckets one by one
 #pragma unroll 1
-            for (uint32_t m = b; m < e; m += KB) {
-                float4 rr[KB];
-#pragma unroll
-                for (int j = 0; j < KB; ++j) {
-                    rr[j] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(0x7fc00000));  // NaN: no match
-                    if (m + j < e) rr[j] = xps[m + j];
-                }
-                bool pass[KB];
-                float ox[KB], oy[KB], oz[KB], d2[KB];
-#pragma unroll
-                for (int j = 0; j < KB; ++j) {
-                    const uint32_t mj = m + j;
-                    const float hx = mj < b1 ? g0 : (mj < b2 ? g1 : g2);
-                    ox[j] = rr[j].x - pi.x;
-                    oy[j] = rr[j].y - pi.y;
-                    oz[j] = rr[j].z - pi.z;
-                    d2[j] = ox[j] * ox[j] + oy[j] * oy[j] + oz[j] * oz[j];  // glm::dot order
-                    pass[j] = rr[j].w == hx && !(SKIP_SELF && mj == k) && d2[j] < thr;
-                }
-                float4 a[KB];
-#pragma unroll
-                for (int j = 0; j < KB; ++j) {
-                    a[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                    if (AUX && pass[j]) a[j] = aux[m + j];
-                }
-#pragma unroll
-                for (int j = 0; j < KB; ++j)
-                    if (pass[j]) body(m + j, ox[j], oy[j], oz[j], d2[j], a[j]);
+            for (int q = 0; q < 3; ++q) {
+                const bool nq = q == 0 ? n0 : (q == 1 ? n1 : n2);
+                const uint32_t b = nq ? (q == 0 ? s0 : (q == 1 ? s1 : s2)) : 0u;
+                const uint32_t e = nq ? (q == 0 ? e0 : (q == 1 ? e1 : e2)) : 0u;
+                const float fq = q == 0 ? f0 : (q == 1 ? f1 : f2);
+                walk_run<false, SKIP_SELF, AUX>(xps, 0, xps, aux, k, pi, thr, b, e, fq, fq, fq, body);
             }
         }
     }
@@ -472,9 +525,9 @@ __device__ __forceinline__ void walk(const SceneDesc &S, const float4 *__restric
                                      const float4 *__restrict__ aux,
                                      const uint32_t *__restrict__ start,
                                      const uint32_t *__restrict__ endt, uint32_t k,
-                                     const float4 &pi, Body &&body)
+                                     const float4 &pi, float4 *stage, Body &&body)
 {
-    if constexpr (GRID) walk_grid<SKIP_SELF, AUX>(S, xps, aux, start, endt, k, pi, body);
+    if constexpr (GRID) walk_grid<SKIP_SELF, AUX>(S, xps, aux, start, endt, k, pi, stage, body);
     else walk_hash<SKIP_SELF, AUX>(S, xps, aux, start, endt, k, pi, body);
 }
 
@@ -520,15 +573,15 @@ __device__ __forceinline__ void gather(const SceneDesc &S, const float4 *__restr
                                        const float4 *__restrict__ aux,
                                        const uint32_t *__restrict__ start,
                                        const uint32_t *__restrict__ endt, uint32_t *lst,
-                                       uint32_t k, const float4 &pi, Body &&body)
+                                       float4 *stage, uint32_t k, const float4 &pi, Body &&body)
 {
     if (!LIST) {
-        walk<GRID, SKIP_SELF, AUX>(S, xps, aux, start, endt, k, pi, body);
+        walk<GRID, SKIP_SELF, AUX>(S, xps, aux, start, endt, k, pi, stage, body);
         return;
     }
     const uint32_t stride = blockDim.x;
     uint32_t c = 0;
-    walk<GRID, SKIP_SELF, false>(S, xps, nullptr, start, endt, k, pi,
+    walk<GRID, SKIP_SELF, false>(S, xps, nullptr, start, endt, k, pi, stage,
                            [&](uint32_t m, float, float, float, float, const float4 &) {
                                if (c < kListMax) lst[c * stride] = m;
                                ++c;
@@ -560,7 +613,7 @@ __device__ __forceinline__ void gather(const SceneDesc &S, const float4 *__restr
     }
     if (c > kListMax) {
         uint32_t skip = kListMax;
-        walk<GRID, SKIP_SELF, AUX>(S, xps, aux, start, endt, k, pi,
+        walk<GRID, SKIP_SELF, AUX>(S, xps, aux, start, endt, k, pi, stage,
                              [&](uint32_t m, float ox, float oy, float oz, float d2,
                                  const float4 &a) {
                                  if (skip) --skip;
@@ -578,6 +631,10 @@ __device__ __forceinline__ void gather(const SceneDesc &S, const float4 *__restr
     const SceneDesc &S = scene_of_slot<MULTI>(sd, scene_of, k);                     \
     const float4 pi = xps[k];                                                       \
     uint32_t *lst = lds_list + threadIdx.x;                                         \
+    /* GRID: each wave's row-staging buffer follows the lists */                    \
+    float4 *stage = GRID ? reinterpret_cast<float4 *>(lds_list + (LIST ? kListMax * blockDim.x : 0)) + \
+                               (threadIdx.x >> 6) * kStageBuf                       \
+                         : nullptr;                                                 \
     (void)g;                                                                        \
     (void)lst
 
@@ -607,7 +664,7 @@ __device__ __forceinline__ void gather_row(const SceneDesc &S, const float4 *__r
                                            uint32_t k, const float4 &pi, Body &&body)
 {
     if (c > kNbrMax) {
-        walk<false, true, true>(S, xps, aux, start, endt, k, pi, body);
+        walk<false, true, true>(S, xps, aux, start, endt, k, pi, nullptr, body);
         return;
     }
     const uint4 *row4 = reinterpret_cast<const uint4 *>(row);
@@ -641,7 +698,7 @@ __global__ __launch_bounds__(256) void k_density(SPH_GATHER_PARAMS, float4 *__re
     float rho = 0.0f, rhon = 0.0f;
     uint32_t *row = NBR ? nbr + (size_t)i * kNbrMax : nullptr;
     uint32_t c = 0;
-    gather<GRID, LIST, false, false>(S, xps, nullptr, start, endt, lst, k, pi,
+    gather<GRID, LIST, false, false>(S, xps, nullptr, start, endt, lst, stage, k, pi,
                                [&](uint32_t m, float, float, float, float d2, const float4 &) {
         const float d = sqrtf(d2);
         const float v = h - d;
@@ -680,7 +737,7 @@ __global__ __launch_bounds__(256) void k_density_rows(SPH_GATHER_PARAMS, float4 
         const float4 pi = xps[k];
         const float h = S.h, w2c = S.w2c, w3c = S.w3c;
         float rho = 0.0f, rhon = 0.0f;
-        walk<false, false, false>(S, xps, nullptr, start, endt, k, pi,
+        walk<false, false, false>(S, xps, nullptr, start, endt, k, pi, nullptr,
                            [&](uint32_t m, float, float, float, float d2, const float4 &) {
             const float d = sqrtf(d2);
             const float v = h - d;
@@ -737,7 +794,7 @@ __global__ __launch_bounds__(256) void k_pressure(SPH_GATHER_PARAMS, const float
         fz += dirz * b * snp / dj.y;
     };
     if (NBR) gather_row(S, xps, dens, start, endt, nbr + (size_t)i * kNbrMax, ncnt[i], k, pi, body);
-    else gather<GRID, LIST, true, true>(S, xps, dens, start, endt, lst, k, pi, body);
+    else gather<GRID, LIST, true, true>(S, xps, dens, start, endt, lst, stage, k, pi, body);
     float4 v = vs[k];
     const float dt = S.dt, rho = di.x;
     v.x = v.x + (fx / rho) * dt;  // cc:421
@@ -765,7 +822,7 @@ __global__ __launch_bounds__(256) void k_viscosity_integrate(SPH_GATHER_PARAMS,
         fz += (vj.z - vi.z) * w;
     };
     if (NBR) gather_row(S, xps, vs2, start, endt, nbr + (size_t)i * kNbrMax, ncnt[i], k, pi, body);
-    else gather<GRID, LIST, true, true>(S, xps, vs2, start, endt, lst, k, pi, body);
+    else gather<GRID, LIST, true, true>(S, xps, vs2, start, endt, lst, stage, k, pi, body);
     const float dt = S.dt, mu = S.mu;
     float4 v = vi;
     v.x = vi.x + fx * mu * dt;  // cc:463
@@ -962,10 +1019,11 @@ inline uint32_t gather_grid(uint32_t cap, int block)
         const size_t lds_ = v_.list ? sizeof(uint32_t) * kListMax * (size_t)b_ : 0;           \
         const bool m_ = s.multi;                                                              \
         if (s.grid) {                                                                         \
-            if (m_ && v_.list) KER<true, false, true, false, true><<<g_, b_, lds_, st>>>(__VA_ARGS__);   \
-            else if (m_) KER<true, false, false, false, true><<<g_, b_, lds_, st>>>(__VA_ARGS__);        \
-            else if (v_.list) KER<false, false, true, false, true><<<g_, b_, lds_, st>>>(__VA_ARGS__);   \
-            else KER<false, false, false, false, true><<<g_, b_, lds_, st>>>(__VA_ARGS__);               \
+            const size_t ldsg_ = lds_ + sizeof(float4) * kStageBuf * (size_t)(b_ / 64);       \
+            if (m_ && v_.list) KER<true, false, true, false, true><<<g_, b_, ldsg_, st>>>(__VA_ARGS__);  \
+            else if (m_) KER<true, false, false, false, true><<<g_, b_, ldsg_, st>>>(__VA_ARGS__);       \
+            else if (v_.list) KER<false, false, true, false, true><<<g_, b_, ldsg_, st>>>(__VA_ARGS__);  \
+            else KER<false, false, false, false, true><<<g_, b_, ldsg_, st>>>(__VA_ARGS__);              \
         } else if (c.nbr && v_.rows) {                                                        \
             if (m_) KER<true, true, false, true, false><<<g_, b_, 0, st>>>(__VA_ARGS__);      \
             else KER<false, true, false, true, false><<<g_, b_, 0, st>>>(__VA_ARGS__);        \

@@ -10,7 +10,10 @@ velocities, densities and neighbour lists: grids large enough for no cell to
 wrap, grids so small that far cells share buckets, and degenerate periods
 (G = 1 or 2 on an axis, where the three cells of a neighbour row wrap onto
 each other).  They also pin the oracle's grid key against a numpy
-restatement.
+restatement, and check a Python restatement of the GPU's row walk (three
+z-adjacent buckets walked as one run, a record passing when its float hash
+equals any of the row's three cell hashes) against the reference walk's
+neighbour lists, with and without wrapping.
 """
 import numpy as np
 import pytest
@@ -131,3 +134,79 @@ def test_neighbour_cells_never_float_alias():
     base = rng.integers(0, m, 10000, dtype=np.uint64)
     other = (base + np.uint64(worst)) % np.uint64(m)
     assert not np.any(base.astype(np.float32) == other.astype(np.float32))
+
+
+def _row_walk_lists(xp, p, gdim, lo, T):
+    """Python restatement of the GPU's grid-key walk (sph_kernels.hip,
+    walk_grid / walk_run): 9 rows (dx outer, dy inner); where the three
+    z-cells of a row are consecutive buckets their slots form one run and a
+    record passes if d2 < thr and its float hash equals ANY of the three row
+    cells' float hashes; at the wrap seam the buckets are walked one by one.
+    Returns per-particle lists of passing particle indices, in walk order,
+    self included, with the prefilter d2 <= r2 only (the neighbour list)."""
+    h = np.float32(p.h)
+    with O.grid_key(gdim, lo):
+        s = O.spatial(xp, T, p.h)
+    n = xp.shape[0]
+    start = s.start
+    end = np.zeros(T, np.int64)
+    counts = np.bincount(s.key_id, minlength=T)
+    nz = counts > 0
+    end[nz] = start[nz].astype(np.int64) + counts[nz]
+    srt = s.sorted_idx
+    rec = xp[srt, :3].astype(np.float32)
+    rech = s.hash_id[srt].astype(np.float32)
+    r2 = np.float32(p.r2)
+    cells = np.floor(xp[:, :3] / h).astype(np.int64)
+    M = 1 << 32
+    G = [int(g) for g in gdim]
+
+    def wrap(u, g):
+        return (u % M) % g
+
+    out = []
+    for i in range(n):
+        c = cells[i]
+        u = [(int(c[a]) - int(lo[a])) % M for a in range(3)]
+        wx = [wrap(u[0] + d - 1, G[0]) for d in range(3)]
+        wy = [wrap(u[1] + d - 1, G[1]) for d in range(3)]
+        wz = [wrap(u[2] + d - 1, G[2]) for d in range(3)]
+        run = wz[1] == wz[0] + 1 and wz[2] == wz[1] + 1
+        h0 = (15823 * c[0] + 9737333 * c[1] + 440817757 * c[2]) % M
+        lst = []
+        pi = xp[i, :3].astype(np.float32)
+        for r in range(9):
+            dx, dy = r // 3 - 1, r % 3 - 1
+            base = (wx[r // 3] * G[1] + wy[r % 3]) * G[2]
+            keys = [base + wz[q] for q in range(3)]
+            hr = (h0 + dx * 15823 + dy * 9737333) % M
+            f = [np.float32((hr + (q - 1) * 440817757) % M) for q in range(3)]
+            if run:
+                segs = []
+                ne = [k for k in keys if start[k] != O.START_EMPTY]
+                if ne:
+                    segs.append((int(start[ne[0]]), int(end[ne[-1]]), f))
+            else:
+                segs = [(int(start[k]), int(end[k]), [f[q]] * 3)
+                        for q, k in enumerate(keys) if start[k] != O.START_EMPTY]
+            for b, e, fs in segs:
+                for m in range(b, e):
+                    o = rec[m] - pi
+                    d2 = np.float32(np.float32(o[0] * o[0]) + np.float32(o[1] * o[1])) + np.float32(o[2] * o[2])
+                    if d2 <= r2 and rech[m] in fs:
+                        lst.append(int(srt[m]))
+        out.append(lst)
+    return out
+
+
+@pytest.mark.parametrize("grid", ["ideal", "wrap", "shifted", "gz2"])
+def test_row_walk_matches_reference_walk(grid):
+    p, pos, vel, dt = _uniform(1500, seed=9)
+    xp = pos.copy()
+    n = xp.shape[0]
+    s = O.spatial(xp, n, p.h)
+    offs, lst = O.neighbours(p, xp, s)
+    want = [list(lst[offs[i]:offs[i + 1]]) for i in range(n)]
+    name, gdim, lo = next(g for g in _grids(p, n) if g[0] == grid)
+    got = _row_walk_lists(xp, p, gdim, lo, max(int(np.prod(gdim)), 1))
+    assert got == want
