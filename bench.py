@@ -54,6 +54,10 @@ def kernel_name(stage, variants, grid, multi=False):
             "viscosity_integrate": "k_viscosity_integrate"}[stage]
     v = variants[stage.split("_")[0]]
     if grid:
+        if stage == "density":
+            return f"{base}<{m}, false, {b(v['list'])}, {b(variants['nbr'])}, true>"
+        if variants["nbr"] and v["rows"]:
+            return f"{base}<{m}, false, false, true, true>"
         return f"{base}<{m}, false, {b(v['list'])}, false, true>"
     if variants["nbr"] and v["rows"]:
         return f"{base}<{m}, true, false, true, false>"

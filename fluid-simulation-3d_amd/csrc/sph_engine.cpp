@@ -621,12 +621,11 @@ int Engine::autotune(float dt, int updates, double ms_out[3], std::string *err)
         bool nbr;
     };
     std::vector<Cand> cands;
-    // (the cell-grid key has neither the work order nor the row mode)
+    // (the cell-grid key has no work order; its row mode uses neighbour masks)
     for (int w = 0; w < (grid_ ? 1 : 2); ++w)
         for (int l = 0; l < 2; ++l)
             for (int b : {64, 128, 256}) cands.push_back({GatherVariant{w != 0, l != 0, b, false}, false});
-    if (!grid_)
-        for (int b : {64, 128, 256}) cands.push_back({GatherVariant{true, false, b, true}, true});
+    for (int b : {64, 128, 256}) cands.push_back({GatherVariant{!grid_, false, b, true}, true});
     double walk_ms[3] = {1e30, 1e30, 1e30}, row_ms[3] = {1e30, 1e30, 1e30};
     GatherVariant walk_v[3] = {keep.density, keep.pressure, keep.viscosity}, row_v[3];
     int rc = check(save(), "autotune snapshot", err);

@@ -122,7 +122,7 @@ class Engine {
     hipError_t enqueue_update(hipEvent_t *ev, bool rebuild_work);
     bool rebuild_due() const
     {
-        return cfg_.needs_work() && (!work_valid_ || work_updates_ % work_period_ == 0);
+        return !grid_ && cfg_.needs_work() && (!work_valid_ || work_updates_ % work_period_ == 0);
     }
     int check(hipError_t e, const char *what, std::string *err);
 

@@ -393,11 +393,11 @@ constexpr uint32_t kStageBuf = kStageCap + KB;  // + padding: a batch may read K
 // branches.  STAGED: the records are read from the wave's LDS copy
 // src[slot - base] of a range containing [b, e) (the padded buffer makes the
 // reads past e harmless; they never pass); else from xps in global memory.
-template <bool STAGED, bool SKIP_SELF, bool AUX, typename Body>
+template <bool STAGED, bool SKIP_SELF, bool AUX, bool MASKS, typename Body>
 __device__ __forceinline__ void walk_run(const float4 *src, uint32_t base, const float4 *__restrict__ xps,
                                          const float4 *__restrict__ aux, uint32_t k,
                                          const float4 &pi, float thr, uint32_t b, uint32_t e,
-                                         float g0, float g1, float g2, Body &body)
+                                         float g0, float g1, float g2, uint64_t &mask, Body &body)
 {
 #pragma unroll 1
     for (uint32_t m = b; m < e; m += KB) {
@@ -432,21 +432,36 @@ __device__ __forceinline__ void walk_run(const float4 *src, uint32_t base, const
             if (AUX && pass[j]) a[j] = aux[m + j];
         }
 #pragma unroll
-        for (int j = 0; j < KB; ++j)
+        for (int j = 0; j < KB; ++j) {
+            if (MASKS && pass[j] && m + j - b < 64u) mask |= 1ull << (m + j - b);
             if (pass[j]) body(m + j, ox[j], oy[j], oz[j], d2[j], a[j]);
+        }
     }
 }
 
-template <bool SKIP_SELF, bool AUX, typename Body>
+// Neighbour masks (grid key): for sorted slot k and row r, the run start
+// rb (base) and a 64-bit mask of the run's slots that passed (self
+// included); kept in the neighbour-row buffer as masks[9][n] (uint2) then
+// bases[9][n]; ok[k] = 0 when a row could not be recorded (a run longer than
+// 64 slots, or a seam row), and the consumer then walks that particle.
+struct MaskOut {
+    uint32_t *buf;  // neighbour-row buffer (>= 27 n words)
+    uint32_t *ok;   // [n]
+    uint32_t n;
+};
+
+template <bool SKIP_SELF, bool AUX, bool MASKS = false, typename Body>
 __device__ __forceinline__ void walk_grid(const SceneDesc &S, const float4 *__restrict__ xps,
                                           const float4 *__restrict__ aux,
                                           const uint32_t *__restrict__ start,
                                           const uint32_t *__restrict__ endt, uint32_t k,
-                                          const float4 &pi, float4 *stage, Body &&body)
+                                          const float4 &pi, float4 *stage, Body &&body,
+                                          const MaskOut &mo = MaskOut{nullptr, nullptr, 0})
 {
     const int cx = cell_coord(pi.x, S.h), cy = cell_coord(pi.y, S.h), cz = cell_coord(pi.z, S.h);
     const float thr = S.thr;
     const uint32_t gy = S.gdim[1], gz = S.gdim[2], key_off = S.key_off;
+    bool masks_ok = true;
     const uint32_t ux = (uint32_t)cx - (uint32_t)S.cell_lo[0];
     const uint32_t uy = (uint32_t)cy - (uint32_t)S.cell_lo[1];
     const uint32_t uz = (uint32_t)cz - (uint32_t)S.cell_lo[2];
@@ -502,11 +517,19 @@ __device__ __forceinline__ void walk_grid(const SceneDesc &S, const float4 *__re
                     staged = true;
                 }
             }
+            uint64_t mask = 0;
             if (staged)
-                walk_run<true, SKIP_SELF, AUX>(stage, lo, xps, aux, k, pi, thr, rb, re, f0, f1, f2, body);
+                walk_run<true, SKIP_SELF, AUX, MASKS>(stage, lo, xps, aux, k, pi, thr, rb, re, f0, f1, f2, mask, body);
             else
-                walk_run<false, SKIP_SELF, AUX>(xps, 0, xps, aux, k, pi, thr, rb, re, f0, f1, f2, body);
+                walk_run<false, SKIP_SELF, AUX, MASKS>(xps, 0, xps, aux, k, pi, thr, rb, re, f0, f1, f2, mask, body);
+            if (MASKS) {
+                masks_ok = masks_ok && re - rb <= 64u;
+                reinterpret_cast<uint2 *>(mo.buf)[(size_t)r * mo.n + k] =
+                    make_uint2((uint32_t)mask, (uint32_t)(mask >> 32));
+                mo.buf[(size_t)(18 + r) * mo.n + k] = rb;
+            }
         } else {
+            masks_ok = false;
             // wrap seam: the three buckets one by one
 #pragma unroll 1
             for (int q = 0; q < 3; ++q) {
@@ -514,20 +537,81 @@ __device__ __forceinline__ void walk_grid(const SceneDesc &S, const float4 *__re
                 const uint32_t b = nq ? (q == 0 ? s0 : (q == 1 ? s1 : s2)) : 0u;
                 const uint32_t e = nq ? (q == 0 ? e0 : (q == 1 ? e1 : e2)) : 0u;
                 const float fq = q == 0 ? f0 : (q == 1 ? f1 : f2);
-                walk_run<false, SKIP_SELF, AUX>(xps, 0, xps, aux, k, pi, thr, b, e, fq, fq, fq, body);
+                uint64_t unused = 0;
+                walk_run<false, SKIP_SELF, AUX, false>(xps, 0, xps, aux, k, pi, thr, b, e, fq, fq, fq, unused, body);
             }
         }
     }
+    if (MASKS) mo.ok[k] = masks_ok ? 1u : 0u;
 }
 
-template <bool GRID, bool SKIP_SELF, bool AUX, typename Body>
+// Consumer side of the neighbour masks: body(m, ox, oy, oz, d2, aux[m]) for
+// every recorded slot m != k, rows in order and slots ascending within a row
+// -- the order of the walk that recorded them.  d2 is recomputed with the
+// walk's expression.  The next row's mask is fetched while a row is used.
+template <typename Body>
+__device__ __forceinline__ void gather_masks(const float4 *__restrict__ xps,
+                                             const float4 *__restrict__ aux,
+                                             const uint32_t *__restrict__ buf, uint32_t n,
+                                             uint32_t k, const float4 &pi, Body &&body)
+{
+    const uint2 *mk = reinterpret_cast<const uint2 *>(buf);
+    const uint32_t *bs = buf + 18 * (size_t)n;
+    uint2 nm = mk[k];
+    uint32_t nb = bs[k];
+    uint64_t cur = 0;
+    uint32_t cb = 0;
+    int r = 0;  // next row to take
+    auto take = [&](uint32_t &m) -> bool {
+        while (cur == 0) {
+            if (r == 9) return false;
+            cur = ((uint64_t)nm.y << 32) | nm.x;
+            cb = nb;
+            ++r;
+            if (r < 9) {
+                nm = mk[(size_t)r * n + k];
+                nb = bs[(size_t)r * n + k];
+            }
+        }
+        m = cb + (uint32_t)__builtin_ctzll(cur);
+        cur &= cur - 1;
+        return true;
+    };
+#pragma unroll 1
+    for (;;) {
+        uint32_t m[KB];
+        bool v[KB];
+#pragma unroll
+        for (int q = 0; q < KB; ++q) v[q] = take(m[q]) && m[q] != k;
+        float4 rr[KB], a[KB];
+#pragma unroll
+        for (int q = 0; q < KB; ++q) {
+            rr[q] = pi;
+            a[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (v[q]) {
+                rr[q] = xps[m[q]];
+                a[q] = aux[m[q]];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < KB; ++q)
+            if (v[q]) {
+                const float ox = rr[q].x - pi.x, oy = rr[q].y - pi.y, oz = rr[q].z - pi.z;
+                body(m[q], ox, oy, oz, ox * ox + oy * oy + oz * oz, a[q]);
+            }
+        if (cur == 0 && r == 9) break;  // every recorded slot has been taken
+    }
+}
+
+template <bool GRID, bool SKIP_SELF, bool AUX, bool MASKS = false, typename Body>
 __device__ __forceinline__ void walk(const SceneDesc &S, const float4 *__restrict__ xps,
                                      const float4 *__restrict__ aux,
                                      const uint32_t *__restrict__ start,
                                      const uint32_t *__restrict__ endt, uint32_t k,
-                                     const float4 &pi, float4 *stage, Body &&body)
+                                     const float4 &pi, float4 *stage, Body &&body,
+                                     const MaskOut &mo = MaskOut{nullptr, nullptr, 0})
 {
-    if constexpr (GRID) walk_grid<SKIP_SELF, AUX>(S, xps, aux, start, endt, k, pi, stage, body);
+    if constexpr (GRID) walk_grid<SKIP_SELF, AUX, MASKS>(S, xps, aux, start, endt, k, pi, stage, body, mo);
     else walk_hash<SKIP_SELF, AUX>(S, xps, aux, start, endt, k, pi, body);
 }
 
@@ -568,24 +652,25 @@ __device__ __forceinline__ uint32_t gather_index()
 // (self excluded when SKIP_SELF), in walk order.  With LIST the candidate
 // tests and the bodies are split into the two phases described above; `lst`
 // is this work-item's column of the block's LDS list (stride blockDim.x).
-template <bool GRID, bool LIST, bool SKIP_SELF, bool AUX, typename Body>
+template <bool GRID, bool LIST, bool SKIP_SELF, bool AUX, bool MASKS = false, typename Body>
 __device__ __forceinline__ void gather(const SceneDesc &S, const float4 *__restrict__ xps,
                                        const float4 *__restrict__ aux,
                                        const uint32_t *__restrict__ start,
                                        const uint32_t *__restrict__ endt, uint32_t *lst,
-                                       float4 *stage, uint32_t k, const float4 &pi, Body &&body)
+                                       float4 *stage, uint32_t k, const float4 &pi, Body &&body,
+                                       const MaskOut &mo = MaskOut{nullptr, nullptr, 0})
 {
     if (!LIST) {
-        walk<GRID, SKIP_SELF, AUX>(S, xps, aux, start, endt, k, pi, stage, body);
+        walk<GRID, SKIP_SELF, AUX, MASKS>(S, xps, aux, start, endt, k, pi, stage, body, mo);
         return;
     }
     const uint32_t stride = blockDim.x;
     uint32_t c = 0;
-    walk<GRID, SKIP_SELF, false>(S, xps, nullptr, start, endt, k, pi, stage,
+    walk<GRID, SKIP_SELF, false, MASKS>(S, xps, nullptr, start, endt, k, pi, stage,
                            [&](uint32_t m, float, float, float, float, const float4 &) {
                                if (c < kListMax) lst[c * stride] = m;
                                ++c;
-                           });
+                           }, mo);
     const uint32_t n1 = min(c, kListMax);
     for (uint32_t j = 0; j < n1; j += KB) {
         uint32_t m[KB];
@@ -696,20 +781,22 @@ __global__ __launch_bounds__(256) void k_density(SPH_GATHER_PARAMS, float4 *__re
     SPH_GATHER_PROLOGUE();
     const float h = S.h, w2c = S.w2c, w3c = S.w3c;
     float rho = 0.0f, rhon = 0.0f;
-    uint32_t *row = NBR ? nbr + (size_t)i * kNbrMax : nullptr;
+    constexpr bool ROWS = NBR && !GRID;   // hash key: neighbour rows
+    constexpr bool MASKS = NBR && GRID;   // grid key: neighbour masks
+    uint32_t *row = ROWS ? nbr + (size_t)i * kNbrMax : nullptr;
     uint32_t c = 0;
-    gather<GRID, LIST, false, false>(S, xps, nullptr, start, endt, lst, stage, k, pi,
+    gather<GRID, LIST, false, false, MASKS>(S, xps, nullptr, start, endt, lst, stage, k, pi,
                                [&](uint32_t m, float, float, float, float d2, const float4 &) {
         const float d = sqrtf(d2);
         const float v = h - d;
         rho += v * v * w2c;       // SmoothingPow2, kernels.h:31
         rhon += v * v * v * w3c;  // SmoothingPow3, kernels.h:43
-        if (NBR && m != k) {
+        if (ROWS && m != k) {
             if (c < kNbrMax) row[c] = m;
             ++c;
         }
-    });
-    if (NBR) ncnt[i] = c;
+    }, MaskOut{nbr, ncnt, ed->n_total});
+    if (ROWS) ncnt[i] = c;
     // pressures of cc:371-372 / 407-408, computed once per particle
     dens[k] = make_float4(rho, rhon, (rho - S.rho0) * S.k, rhon * S.k_near);
 }
@@ -793,8 +880,14 @@ __global__ __launch_bounds__(256) void k_pressure(SPH_GATHER_PARAMS, const float
         fy += diry * b * snp / dj.y;
         fz += dirz * b * snp / dj.y;
     };
-    if (NBR) gather_row(S, xps, dens, start, endt, nbr + (size_t)i * kNbrMax, ncnt[i], k, pi, body);
-    else gather<GRID, LIST, true, true>(S, xps, dens, start, endt, lst, stage, k, pi, body);
+    if (NBR && GRID) {
+        if (ncnt[k]) gather_masks(xps, dens, nbr, ed->n_total, k, pi, body);
+        else gather<GRID, false, true, true>(S, xps, dens, start, endt, lst, stage, k, pi, body);
+    } else if (NBR) {
+        gather_row(S, xps, dens, start, endt, nbr + (size_t)i * kNbrMax, ncnt[i], k, pi, body);
+    } else {
+        gather<GRID, LIST, true, true>(S, xps, dens, start, endt, lst, stage, k, pi, body);
+    }
     float4 v = vs[k];
     const float dt = S.dt, rho = di.x;
     v.x = v.x + (fx / rho) * dt;  // cc:421
@@ -821,8 +914,14 @@ __global__ __launch_bounds__(256) void k_viscosity_integrate(SPH_GATHER_PARAMS,
         fy += (vj.y - vi.y) * w;
         fz += (vj.z - vi.z) * w;
     };
-    if (NBR) gather_row(S, xps, vs2, start, endt, nbr + (size_t)i * kNbrMax, ncnt[i], k, pi, body);
-    else gather<GRID, LIST, true, true>(S, xps, vs2, start, endt, lst, stage, k, pi, body);
+    if (NBR && GRID) {
+        if (ncnt[k]) gather_masks(xps, vs2, nbr, ed->n_total, k, pi, body);
+        else gather<GRID, false, true, true>(S, xps, vs2, start, endt, lst, stage, k, pi, body);
+    } else if (NBR) {
+        gather_row(S, xps, vs2, start, endt, nbr + (size_t)i * kNbrMax, ncnt[i], k, pi, body);
+    } else {
+        gather<GRID, LIST, true, true>(S, xps, vs2, start, endt, lst, stage, k, pi, body);
+    }
     const float dt = S.dt, mu = S.mu;
     float4 v = vi;
     v.x = vi.x + fx * mu * dt;  // cc:463
@@ -1018,7 +1117,11 @@ inline uint32_t gather_grid(uint32_t cap, int block)
         const uint32_t g_ = gather_grid(s.capacity, b_);                                      \
         const size_t lds_ = v_.list ? sizeof(uint32_t) * kListMax * (size_t)b_ : 0;           \
         const bool m_ = s.multi;                                                              \
-        if (s.grid) {                                                                         \
+        if (s.grid && c.nbr && v_.rows) {                                                     \
+            const size_t ldsr_ = sizeof(float4) * kStageBuf * (size_t)(b_ / 64);              \
+            if (m_) KER<true, false, false, true, true><<<g_, b_, ldsr_, st>>>(__VA_ARGS__);  \
+            else KER<false, false, false, true, true><<<g_, b_, ldsr_, st>>>(__VA_ARGS__);    \
+        } else if (s.grid) {                                                                  \
             const size_t ldsg_ = lds_ + sizeof(float4) * kStageBuf * (size_t)(b_ / 64);       \
             if (m_ && v_.list) KER<true, false, true, false, true><<<g_, b_, ldsg_, st>>>(__VA_ARGS__);  \
             else if (m_) KER<true, false, false, false, true><<<g_, b_, ldsg_, st>>>(__VA_ARGS__);       \
@@ -1043,6 +1146,22 @@ inline uint32_t gather_grid(uint32_t cap, int block)
 
 hipError_t launch_density(const DevState &s, const LaunchCfg &c, int buf, hipStream_t st)
 {
+    if (c.nbr && s.grid) {
+        // the density pass records the neighbour masks
+        const int b = c.density.block;
+        const uint32_t g = gather_grid(s.capacity, b);
+        const size_t lds = (c.density.list ? sizeof(uint32_t) * kListMax * (size_t)b : 0) +
+                           sizeof(float4) * kStageBuf * (size_t)(b / 64);
+        if (s.multi && c.density.list)
+            k_density<true, false, true, true, true><<<g, b, lds, st>>>(SPH_GATHER_ARGS, s.dens);
+        else if (s.multi)
+            k_density<true, false, false, true, true><<<g, b, lds, st>>>(SPH_GATHER_ARGS, s.dens);
+        else if (c.density.list)
+            k_density<false, false, true, true, true><<<g, b, lds, st>>>(SPH_GATHER_ARGS, s.dens);
+        else
+            k_density<false, false, false, true, true><<<g, b, lds, st>>>(SPH_GATHER_ARGS, s.dens);
+        SPH_LAUNCH_CHECK();
+    }
     if (c.nbr && !s.grid) {
         const int b = c.density.block;
         const size_t lds = sizeof(uint32_t) * ((size_t)b * kRowPitch + b);

@@ -64,17 +64,12 @@ struct LaunchCfg {
 };
 
 // Defaults per bucket key.  Under the cell-grid key the sorted order is
-// already spatially coherent: the kernels run in it (no Morton work order,
-// no neighbour rows).
+// already spatially coherent: the kernels run in it (no Morton work order),
+// and the row mode records neighbour masks instead of neighbour rows.
 inline LaunchCfg sanitize_launch_cfg(LaunchCfg c, bool grid)
 {
-    if (grid) {
-        for (GatherVariant *v : {&c.density, &c.pressure, &c.viscosity}) {
-            v->work = false;
-            v->rows = false;
-        }
-        c.nbr = false;
-    }
+    if (grid)
+        for (GatherVariant *v : {&c.density, &c.pressure, &c.viscosity}) v->work = false;
     return c;
 }
 

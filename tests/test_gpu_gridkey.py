@@ -120,29 +120,44 @@ def test_grid_key_neighbour_lists(case):
     np.testing.assert_array_equal(list_g, list_o)
 
 
-@pytest.mark.parametrize("case", ["dam_break_16k", "outside_box_wrap"])
+def _dense():
+    """~97 neighbours per particle: row runs longer than 64 slots, so the
+    neighbour masks cannot hold them and those particles walk instead."""
+    n = 8192
+    pos, vel = S.generate(S.INIT_DAM_BREAK, 3, n, (4.0, 4.0, 4.0), 0.07, 0.001)
+    return W.Scene("dense_8192", W.scene_params(), n, n, pos, vel, 1)
+
+
+# (list, block, neighbour masks, pressure reads masks, viscosity reads masks)
+GRID_VARIANTS = [(l, b, 0, 0, 0) for l in (0, 1) for b in (64, 128, 256)] + \
+    [(l, b, 1, 1, 1) for l in (0, 1) for b in (64, 256)] + [(0, 128, 1, 1, 0), (1, 128, 1, 0, 1)]
+
+
+@pytest.mark.parametrize("case", ["dam_break_16k", "outside_box_wrap", "dense"])
 def test_grid_key_every_variant_bitexact(case):
-    sc = _cases()[case]()
+    sc = _dense() if case == "dense" else _cases()[case]()
     dt = _sub_dt()
     ref = _oracle(sc, dt)
     with S.Sim(sc.params, sc.n, S.TABLE_GRID) as sim:
-        for l in (0, 1):
-            for b in (64, 128, 256):
-                announce("grid variant", case, "list", l, "block", b)
-                v = {"work": 0, "list": l, "block": b, "rows": False}
-                sim.variants = {"density": v, "pressure": v, "viscosity": v, "nbr": 0}
-                sim.upload(sc.pos, sc.vel)
-                sim.update(dt)
-                pos_g, vel_g, dens_g = sim.download()
-                assert_bitexact(dens_g, ref["dens"], f"density {v}")
-                assert_bitexact(vel_g[:, :3], ref["vel"][:, :3], f"velocity {v}")
-                assert_bitexact(pos_g[:, :3], ref["pos"][:, :3], f"position {v}")
-        # variants the grid key does not have are switched off
-        sim.variants = {"density": {"work": 1, "list": 0, "block": 64, "rows": True},
-                        "pressure": {"work": 1, "list": 1, "block": 64, "rows": True},
-                        "viscosity": {"work": 0, "list": 0, "block": 64, "rows": False}, "nbr": 1}
+        for (l, b, nb, rp, rv) in GRID_VARIANTS:
+            announce("grid variant", case, "list", l, "block", b, "masks", nb, rp, rv)
+            v = {"work": 0, "list": l, "block": b, "rows": False}
+            sim.variants = {"density": v, "pressure": dict(v, rows=bool(rp)),
+                            "viscosity": dict(v, rows=bool(rv)), "nbr": nb}
+            got = sim.variants
+            assert got["nbr"] == bool(nb) and got["pressure"]["rows"] == bool(rp)
+            sim.upload(sc.pos, sc.vel)
+            sim.update(dt)
+            pos_g, vel_g, dens_g = sim.download()
+            assert_bitexact(dens_g, ref["dens"], f"density {v} masks {nb}")
+            assert_bitexact(vel_g[:, :3], ref["vel"][:, :3], f"velocity {v} masks {nb}{rp}{rv}")
+            assert_bitexact(pos_g[:, :3], ref["pos"][:, :3], f"position {v} masks {nb}{rp}{rv}")
+        # the grid key has no Morton work order
+        sim.variants = {"density": {"work": 1, "list": 0, "block": 64, "rows": False},
+                        "pressure": {"work": 1, "list": 1, "block": 64, "rows": False},
+                        "viscosity": {"work": 0, "list": 0, "block": 64, "rows": False}, "nbr": 0}
         got = sim.variants
-        assert not got["nbr"] and not got["density"]["work"] and not got["pressure"]["rows"]
+        assert not got["density"]["work"] and not got["pressure"]["work"]
 
 
 def test_grid_key_rollout_and_autotune():
@@ -183,9 +198,9 @@ def test_grid_key_batch_equals_oracle():
     with S.Batch([sc.params for sc in scenes], [sc.n for sc in scenes]) as b:
         for s, sc in enumerate(scenes):
             b.upload(s, sc.pos, sc.vel)
-        for l in (0, 1):
-            v = {"work": 0, "list": l, "block": 128, "rows": False}
-            b.variants = {"density": v, "pressure": v, "viscosity": v, "nbr": 0}
+        for l, nb in ((0, 0), (1, 0), (0, 1)):
+            v = {"work": 0, "list": l, "block": 128, "rows": bool(nb)}
+            b.variants = {"density": v, "pressure": v, "viscosity": v, "nbr": nb}
             for s, sc in enumerate(scenes):
                 b.upload(s, sc.pos, sc.vel)
             b.step(W.DT, W.SUBSTEPS)
@@ -193,6 +208,6 @@ def test_grid_key_batch_equals_oracle():
             for s, sc in enumerate(scenes):
                 pos_o, vel_o, dens_o = O.update(orc_params(sc.params), _sub_dt(), sc.pos, sc.vel,
                                                 T=sc.table, substeps=W.SUBSTEPS)
-                assert_bitexact(got[s][0][:, :3], pos_o[:, :3], f"scene {s} pos list {l}")
-                assert_bitexact(got[s][1][:, :3], vel_o[:, :3], f"scene {s} vel list {l}")
-                assert_bitexact(got[s][2], dens_o, f"scene {s} dens list {l}")
+                assert_bitexact(got[s][0][:, :3], pos_o[:, :3], f"scene {s} pos list {l} masks {nb}")
+                assert_bitexact(got[s][1][:, :3], vel_o[:, :3], f"scene {s} vel list {l} masks {nb}")
+                assert_bitexact(got[s][2], dens_o, f"scene {s} dens list {l} masks {nb}")
