@@ -546,32 +546,41 @@ __device__ __forceinline__ void walk_grid(const SceneDesc &S, const float4 *__re
 }
 
 // Consumer side of the neighbour masks: body(m, ox, oy, oz, d2, aux[m]) for
-// every recorded slot m != k, rows in order and slots ascending within a row
-// -- the order of the walk that recorded them.  d2 is recomputed with the
-// walk's expression.  The next row's mask is fetched while a row is used.
-template <typename Body>
+// every recorded slot m (m != k when SKIP_SELF), rows in order and slots
+// ascending within a row -- the order of the walk that recorded them.  d2 is
+// recomputed with the walk's expression.  The nine rows' masks and bases are
+// loaded at once into the wave's LDS table `tab` (rows x 64 lanes, 3 words
+// each), so stepping to the next row costs an LDS read; NB neighbours'
+// records are loaded per round.
+constexpr uint32_t kMaskTabWords = 9 * 64 * 3;  // per wave
+
+template <int NB, bool SKIP_SELF, bool AUX, typename Body>
 __device__ __forceinline__ void gather_masks(const float4 *__restrict__ xps,
                                              const float4 *__restrict__ aux,
                                              const uint32_t *__restrict__ buf, uint32_t n,
-                                             uint32_t k, const float4 &pi, Body &&body)
+                                             uint32_t k, const float4 &pi, uint32_t *tab,
+                                             Body &&body)
 {
     const uint2 *mk = reinterpret_cast<const uint2 *>(buf);
     const uint32_t *bs = buf + 18 * (size_t)n;
-    uint2 nm = mk[k];
-    uint32_t nb = bs[k];
+    const uint32_t lane = threadIdx.x & 63u;
+    uint2 *tm = reinterpret_cast<uint2 *>(tab);  // [9][64] masks
+    uint32_t *tb = tab + 9 * 64 * 2;             // [9][64] bases
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        tm[r * 64 + lane] = mk[(size_t)r * n + k];
+        tb[r * 64 + lane] = bs[(size_t)r * n + k];
+    }
     uint64_t cur = 0;
     uint32_t cb = 0;
-    int r = 0;  // next row to take
+    uint32_t r = 0;  // next row to take
     auto take = [&](uint32_t &m) -> bool {
         while (cur == 0) {
             if (r == 9) return false;
-            cur = ((uint64_t)nm.y << 32) | nm.x;
-            cb = nb;
+            const uint2 w = tm[r * 64 + lane];
+            cur = ((uint64_t)w.y << 32) | w.x;
+            cb = tb[r * 64 + lane];
             ++r;
-            if (r < 9) {
-                nm = mk[(size_t)r * n + k];
-                nb = bs[(size_t)r * n + k];
-            }
         }
         m = cb + (uint32_t)__builtin_ctzll(cur);
         cur &= cur - 1;
@@ -579,22 +588,22 @@ __device__ __forceinline__ void gather_masks(const float4 *__restrict__ xps,
     };
 #pragma unroll 1
     for (;;) {
-        uint32_t m[KB];
-        bool v[KB];
+        uint32_t m[NB];
+        bool v[NB];
 #pragma unroll
-        for (int q = 0; q < KB; ++q) v[q] = take(m[q]) && m[q] != k;
-        float4 rr[KB], a[KB];
+        for (int q = 0; q < NB; ++q) v[q] = take(m[q]) && !(SKIP_SELF && m[q] == k);
+        float4 rr[NB], a[NB];
 #pragma unroll
-        for (int q = 0; q < KB; ++q) {
+        for (int q = 0; q < NB; ++q) {
             rr[q] = pi;
             a[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if (v[q]) {
                 rr[q] = xps[m[q]];
-                a[q] = aux[m[q]];
+                if (AUX) a[q] = aux[m[q]];
             }
         }
 #pragma unroll
-        for (int q = 0; q < KB; ++q)
+        for (int q = 0; q < NB; ++q)
             if (v[q]) {
                 const float ox = rr[q].x - pi.x, oy = rr[q].y - pi.y, oz = rr[q].z - pi.z;
                 body(m[q], ox, oy, oz, ox * ox + oy * oy + oz * oz, a[q]);
@@ -720,6 +729,11 @@ __device__ __forceinline__ void gather(const SceneDesc &S, const float4 *__restr
     float4 *stage = GRID ? reinterpret_cast<float4 *>(lds_list + (LIST ? kListMax * blockDim.x : 0)) + \
                                (threadIdx.x >> 6) * kStageBuf                       \
                          : nullptr;                                                 \
+    /* GRID + NBR consumers: each wave's neighbour-mask table follows */           \
+    uint32_t *mtab = reinterpret_cast<uint32_t *>(reinterpret_cast<float4 *>(       \
+                         lds_list + (LIST ? kListMax * blockDim.x : 0)) +           \
+                         (blockDim.x >> 6) * kStageBuf) + (threadIdx.x >> 6) * kMaskTabWords; \
+    (void)mtab;                                                                     \
     (void)g;                                                                        \
     (void)lst
 
@@ -785,8 +799,7 @@ __global__ __launch_bounds__(256) void k_density(SPH_GATHER_PARAMS, float4 *__re
     constexpr bool MASKS = NBR && GRID;   // grid key: neighbour masks
     uint32_t *row = ROWS ? nbr + (size_t)i * kNbrMax : nullptr;
     uint32_t c = 0;
-    gather<GRID, LIST, false, false, MASKS>(S, xps, nullptr, start, endt, lst, stage, k, pi,
-                               [&](uint32_t m, float, float, float, float d2, const float4 &) {
+    auto body = [&](uint32_t m, float, float, float, float d2, const float4 &) {
         const float d = sqrtf(d2);
         const float v = h - d;
         rho += v * v * w2c;       // SmoothingPow2, kernels.h:31
@@ -795,7 +808,10 @@ __global__ __launch_bounds__(256) void k_density(SPH_GATHER_PARAMS, float4 *__re
             if (c < kNbrMax) row[c] = m;
             ++c;
         }
-    }, MaskOut{nbr, ncnt, ed->n_total});
+    };
+    // MASKS: the walk also records the neighbour masks
+    gather<GRID, LIST, false, false, MASKS>(S, xps, nullptr, start, endt, lst, stage, k, pi, body,
+                                            MaskOut{nbr, ncnt, ed->n_total});
     if (ROWS) ncnt[i] = c;
     // pressures of cc:371-372 / 407-408, computed once per particle
     dens[k] = make_float4(rho, rhon, (rho - S.rho0) * S.k, rhon * S.k_near);
@@ -881,7 +897,7 @@ __global__ __launch_bounds__(256) void k_pressure(SPH_GATHER_PARAMS, const float
         fz += dirz * b * snp / dj.y;
     };
     if (NBR && GRID) {
-        if (ncnt[k]) gather_masks(xps, dens, nbr, ed->n_total, k, pi, body);
+        if (ncnt[k]) gather_masks<KB, true, true>(xps, dens, nbr, ed->n_total, k, pi, mtab, body);
         else gather<GRID, false, true, true>(S, xps, dens, start, endt, lst, stage, k, pi, body);
     } else if (NBR) {
         gather_row(S, xps, dens, start, endt, nbr + (size_t)i * kNbrMax, ncnt[i], k, pi, body);
@@ -904,6 +920,7 @@ __global__ __launch_bounds__(256) void k_viscosity_integrate(SPH_GATHER_PARAMS,
 {
     SPH_GATHER_PROLOGUE();
     const float4 vi = vs2[k];
+    const float4 x0 = pos[g];  // issued early: used after the gather
     const float hh = S.hh, c6 = S.poly6c;
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     auto body = [&](uint32_t, float, float, float, float d2, const float4 &vj) {
@@ -915,7 +932,7 @@ __global__ __launch_bounds__(256) void k_viscosity_integrate(SPH_GATHER_PARAMS,
         fz += (vj.z - vi.z) * w;
     };
     if (NBR && GRID) {
-        if (ncnt[k]) gather_masks(xps, vs2, nbr, ed->n_total, k, pi, body);
+        if (ncnt[k]) gather_masks<8, true, true>(xps, vs2, nbr, ed->n_total, k, pi, mtab, body);
         else gather<GRID, false, true, true>(S, xps, vs2, start, endt, lst, stage, k, pi, body);
     } else if (NBR) {
         gather_row(S, xps, vs2, start, endt, nbr + (size_t)i * kNbrMax, ncnt[i], k, pi, body);
@@ -928,7 +945,7 @@ __global__ __launch_bounds__(256) void k_viscosity_integrate(SPH_GATHER_PARAMS,
     v.y = vi.y + fy * mu * dt;
     v.z = vi.z + fz * mu * dt;
     // stage F (cc:84-107), per particle
-    float4 x = pos[g];
+    float4 x = x0;
     x.x += v.x * dt;
     x.y += v.y * dt;
     x.z += v.z * dt;
@@ -1118,7 +1135,8 @@ inline uint32_t gather_grid(uint32_t cap, int block)
         const size_t lds_ = v_.list ? sizeof(uint32_t) * kListMax * (size_t)b_ : 0;           \
         const bool m_ = s.multi;                                                              \
         if (s.grid && c.nbr && v_.rows) {                                                     \
-            const size_t ldsr_ = sizeof(float4) * kStageBuf * (size_t)(b_ / 64);              \
+            const size_t ldsr_ = (sizeof(float4) * kStageBuf + sizeof(uint32_t) * kMaskTabWords) * \
+                                 (size_t)(b_ / 64);                                           \
             if (m_) KER<true, false, false, true, true><<<g_, b_, ldsr_, st>>>(__VA_ARGS__);  \
             else KER<false, false, false, true, true><<<g_, b_, ldsr_, st>>>(__VA_ARGS__);    \
         } else if (s.grid) {                                                                  \
