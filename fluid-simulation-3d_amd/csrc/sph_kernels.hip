@@ -380,8 +380,9 @@ __device__ __forceinline__ void walk_hash(const SceneDesc &S, const float4 *__re
 // buffer with coalesced loads and the lanes walk their runs from LDS.  The
 // check is exact per lane, so the range assumption never affects a result;
 // rows that fail it are walked from global memory.
-constexpr uint32_t kStageCap = 120;            // staged records per wave and row
-constexpr uint32_t kStageBuf = kStageCap + KB;  // + padding: a batch may read KB-1 past the end
+constexpr int KBG = 4;                          // records per batch of the grid walk
+constexpr uint32_t kStageCap = 120;             // staged records per wave and row
+constexpr uint32_t kStageBuf = kStageCap + KBG; // + padding: a batch may read KBG-1 past the end
 
 // Slots [b, e) of a grid row: the buckets of the row's cells dz = -1, 0, 1
 // (float hashes g0, g1, g2), in that order.  A record passes when d2 < thr
@@ -400,10 +401,10 @@ __device__ __forceinline__ void walk_run(const float4 *src, uint32_t base, const
                                          float g0, float g1, float g2, uint64_t &mask, Body &body)
 {
 #pragma unroll 1
-    for (uint32_t m = b; m < e; m += KB) {
-        float4 rr[KB];
+    for (uint32_t m = b; m < e; m += KBG) {
+        float4 rr[KBG];
 #pragma unroll
-        for (int j = 0; j < KB; ++j) {
+        for (int j = 0; j < KBG; ++j) {
             if (STAGED) {
                 rr[j] = src[m + j - base];
             } else {
@@ -411,10 +412,10 @@ __device__ __forceinline__ void walk_run(const float4 *src, uint32_t base, const
                 if (m + j < e) rr[j] = xps[m + j];
             }
         }
-        bool pass[KB];
-        float ox[KB], oy[KB], oz[KB], d2[KB];
+        bool pass[KBG];
+        float ox[KBG], oy[KBG], oz[KBG], d2[KBG];
 #pragma unroll
-        for (int j = 0; j < KB; ++j) {
+        for (int j = 0; j < KBG; ++j) {
             const uint32_t mj = m + j;
             ox[j] = rr[j].x - pi.x;
             oy[j] = rr[j].y - pi.y;
@@ -425,14 +426,14 @@ __device__ __forceinline__ void walk_run(const float4 *src, uint32_t base, const
                        ((int)(w == g0) | (int)(w == g1) | (int)(w == g2)) &
                        (int)(!SKIP_SELF || mj != k)) != 0;
         }
-        float4 a[KB];
+        float4 a[KBG];
 #pragma unroll
-        for (int j = 0; j < KB; ++j) {
+        for (int j = 0; j < KBG; ++j) {
             a[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if (AUX && pass[j]) a[j] = aux[m + j];
         }
 #pragma unroll
-        for (int j = 0; j < KB; ++j) {
+        for (int j = 0; j < KBG; ++j) {
             if (MASKS && pass[j] && m + j - b < 64u) mask |= 1ull << (m + j - b);
             if (pass[j]) body(m + j, ox[j], oy[j], oz[j], d2[j], a[j]);
         }
@@ -450,7 +451,12 @@ struct MaskOut {
     uint32_t n;
 };
 
-template <bool SKIP_SELF, bool AUX, bool MASKS = false, typename Body>
+// DEFER (density): where a row's run fits the 64-bit mask, the walk over the
+// run only records the mask, and the body then runs over the mask's slots
+// in ascending order (the walk order), reading the records from the wave's
+// LDS copy of the row when it was staged.  The body then costs one pass per
+// neighbour instead of one per tested candidate batch.
+template <bool SKIP_SELF, bool AUX, bool MASKS = false, bool DEFER = false, typename Body>
 __device__ __forceinline__ void walk_grid(const SceneDesc &S, const float4 *__restrict__ xps,
                                           const float4 *__restrict__ aux,
                                           const uint32_t *__restrict__ start,
@@ -518,10 +524,39 @@ __device__ __forceinline__ void walk_grid(const SceneDesc &S, const float4 *__re
                 }
             }
             uint64_t mask = 0;
+            const bool defer = DEFER && re - rb <= 64u;
+            auto inline_body = [&](uint32_t m, float ox, float oy, float oz, float d2, const float4 &a) {
+                if (!defer) body(m, ox, oy, oz, d2, a);
+            };
             if (staged)
-                walk_run<true, SKIP_SELF, AUX, MASKS>(stage, lo, xps, aux, k, pi, thr, rb, re, f0, f1, f2, mask, body);
+                walk_run<true, SKIP_SELF, AUX, MASKS || DEFER>(stage, lo, xps, aux, k, pi, thr, rb, re, f0, f1, f2,
+                                                              mask, inline_body);
             else
-                walk_run<false, SKIP_SELF, AUX, MASKS>(xps, 0, xps, aux, k, pi, thr, rb, re, f0, f1, f2, mask, body);
+                walk_run<false, SKIP_SELF, AUX, MASKS || DEFER>(xps, 0, xps, aux, k, pi, thr, rb, re, f0, f1, f2,
+                                                               mask, inline_body);
+            if (DEFER && defer) {
+                uint64_t mm = mask;
+                const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (staged) {
+#pragma unroll 1
+                    while (mm) {
+                        const uint32_t m = rb + (uint32_t)__builtin_ctzll(mm);
+                        mm &= mm - 1;
+                        const float4 rec = stage[m - lo];
+                        const float ox = rec.x - pi.x, oy = rec.y - pi.y, oz = rec.z - pi.z;
+                        body(m, ox, oy, oz, ox * ox + oy * oy + oz * oz, zero);
+                    }
+                } else {
+#pragma unroll 1
+                    while (mm) {
+                        const uint32_t m = rb + (uint32_t)__builtin_ctzll(mm);
+                        mm &= mm - 1;
+                        const float4 rec = xps[m];
+                        const float ox = rec.x - pi.x, oy = rec.y - pi.y, oz = rec.z - pi.z;
+                        body(m, ox, oy, oz, ox * ox + oy * oy + oz * oz, zero);
+                    }
+                }
+            }
             if (MASKS) {
                 masks_ok = masks_ok && re - rb <= 64u;
                 reinterpret_cast<uint2 *>(mo.buf)[(size_t)r * mo.n + k] =
@@ -612,7 +647,7 @@ __device__ __forceinline__ void gather_masks(const float4 *__restrict__ xps,
     }
 }
 
-template <bool GRID, bool SKIP_SELF, bool AUX, bool MASKS = false, typename Body>
+template <bool GRID, bool SKIP_SELF, bool AUX, bool MASKS = false, bool DEFER = false, typename Body>
 __device__ __forceinline__ void walk(const SceneDesc &S, const float4 *__restrict__ xps,
                                      const float4 *__restrict__ aux,
                                      const uint32_t *__restrict__ start,
@@ -620,7 +655,7 @@ __device__ __forceinline__ void walk(const SceneDesc &S, const float4 *__restric
                                      const float4 &pi, float4 *stage, Body &&body,
                                      const MaskOut &mo = MaskOut{nullptr, nullptr, 0})
 {
-    if constexpr (GRID) walk_grid<SKIP_SELF, AUX, MASKS>(S, xps, aux, start, endt, k, pi, stage, body, mo);
+    if constexpr (GRID) walk_grid<SKIP_SELF, AUX, MASKS, DEFER>(S, xps, aux, start, endt, k, pi, stage, body, mo);
     else walk_hash<SKIP_SELF, AUX>(S, xps, aux, start, endt, k, pi, body);
 }
 
@@ -809,9 +844,14 @@ __global__ __launch_bounds__(256) void k_density(SPH_GATHER_PARAMS, float4 *__re
             ++c;
         }
     };
-    // MASKS: the walk also records the neighbour masks
-    gather<GRID, LIST, false, false, MASKS>(S, xps, nullptr, start, endt, lst, stage, k, pi, body,
-                                            MaskOut{nbr, ncnt, ed->n_total});
+    // MASKS: the walk also records the neighbour masks.  Grid key without
+    // the LDS list: the body runs over each row's mask (DEFER).
+    if (GRID && !LIST)
+        walk<GRID, false, false, MASKS, true>(S, xps, nullptr, start, endt, k, pi, stage, body,
+                                              MaskOut{nbr, ncnt, ed->n_total});
+    else
+        gather<GRID, LIST, false, false, MASKS>(S, xps, nullptr, start, endt, lst, stage, k, pi, body,
+                                                MaskOut{nbr, ncnt, ed->n_total});
     if (ROWS) ncnt[i] = c;
     // pressures of cc:371-372 / 407-408, computed once per particle
     dens[k] = make_float4(rho, rhon, (rho - S.rho0) * S.k, rhon * S.k_near);
