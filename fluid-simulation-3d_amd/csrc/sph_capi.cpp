@@ -1,5 +1,6 @@
 // sph_capi.cpp -- the C ABI of include/sph_amd.h over sph::Engine.
 #include <hip/hip_runtime.h>
+#include <math.h>
 #include <string.h>
 
 #include <new>
@@ -209,21 +210,56 @@ int sph_upload(sph_handle h, const float *pos4, const float *vel4)
     SPH_GUARD_END
 }
 
-int sph_initialize_data(sph_handle h)
+int sph_initialize(sph_handle h, int kind, uint64_t seed, const float box[3], float spacing,
+                   float jitter, float speed)
 {
     SPH_GUARD_BEGIN
     SPH_NEED(h);
-    const uint32_t n = h->eng->scene(0).n;
-    std::vector<float> pos(4 * (size_t)n), vel(4 * (size_t)n, 0.0f);
-    const float box[3] = {0, 0, 0};
-    int rc = sph_generate(SPH_INIT_GRID_ARRANGEMENT, 0, n, box, 0.0f, 0.0f, 0.0f, pos.data(),
-                          vel.data());
-    if (rc) return rc;
+    sph::GenPlan g;
+    int rc = sph::plan_generate(kind, seed, h->eng->scene(0).n, box, spacing, jitter, speed, &g);
+    if (rc) return fail(rc, "invalid scene generator arguments");
     std::string err;
-    rc = h->eng->upload(0, pos.data(), vel.data(), &err);
+    rc = h->eng->generate(0, g, &err);
     if (!rc) rc = h->eng->prime(&err);
     return done(rc, err);
     SPH_GUARD_END
+}
+
+int sph_initialize_data(sph_handle h)
+{
+    return sph_initialize(h, SPH_INIT_GRID_ARRANGEMENT, 0, nullptr, 0.0f, 0.0f, 0.0f);
+}
+
+int sph_set_colors(sph_handle h, const float stops[16], int enable)
+{
+    SPH_GUARD_BEGIN
+    SPH_NEED(h);
+    if (stops)
+        for (int c = 0; c < 16; ++c)
+            if (!isfinite(stops[c])) return fail(SPH_ERR_INVALID, "colour stops must be finite");
+    std::string err;
+    return done(h->eng->set_colors(stops, enable != 0, &err), err);
+    SPH_GUARD_END
+}
+
+int sph_download_colors(sph_handle h, float *rgba4)
+{
+    SPH_GUARD_BEGIN
+    SPH_NEED(h);
+    std::string err;
+    int rc = h->eng->sync(&err);
+    if (!rc) rc = h->eng->download_colors(0, rgba4, &err);
+    return done(rc, err);
+    SPH_GUARD_END
+}
+
+int sph_render_buffers(sph_handle h, const void **positions, const void **colors)
+{
+    SPH_NEED(h);
+    const uint32_t off = h->eng->scene(0).p_off;
+    if (positions) *positions = h->eng->dev().pos + off;
+    if (colors) *colors = h->eng->colors_buffer() ? h->eng->colors_buffer() + off : nullptr;
+    return SPH_OK;
 }
 
 int sph_update(sph_handle h, float dt)
@@ -415,6 +451,20 @@ int sph_batch_destroy(sph_batch b)
 int sph_batch_upload(sph_batch b, uint32_t scene, const float *pos4, const float *vel4)
 {
     SPH_BATCH_CALL(b->eng->upload(scene, pos4, vel4, &err))
+}
+
+int sph_batch_initialize(sph_batch b, uint32_t scene, int kind, uint64_t seed, const float box[3],
+                         float spacing, float jitter, float speed)
+{
+    SPH_GUARD_BEGIN
+    SPH_NEED(b);
+    if (scene >= b->eng->n_scenes()) return fail(SPH_ERR_INVALID, "scene index out of range");
+    sph::GenPlan g;
+    if (sph::plan_generate(kind, seed, b->eng->scene(scene).n, box, spacing, jitter, speed, &g))
+        return fail(SPH_ERR_INVALID, "invalid scene generator arguments");
+    std::string err;
+    return done(b->eng->generate(scene, g, &err), err);
+    SPH_GUARD_END
 }
 
 int sph_batch_set_params(sph_batch b, uint32_t scene, const sph_params *p)

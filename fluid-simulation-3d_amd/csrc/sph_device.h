@@ -2,6 +2,7 @@
 // gfx950 kernels and by host code (the same source compiles for both, so the
 // host-side unit tests exercise the exact device arithmetic).
 #pragma once
+#include <math.h>
 #include <stdint.h>
 
 #if defined(__HIPCC__)
@@ -49,6 +50,38 @@ SPH_HD uint32_t grid_key(int32_t cx, int32_t cy, int32_t cz, const int32_t lo[3]
     const uint32_t wy = grid_wrap((uint32_t)cy - (uint32_t)lo[1], g[1], m[1]);
     const uint32_t wz = grid_wrap((uint32_t)cz - (uint32_t)lo[2], g[2], m[2]);
     return (wx * g[1] + wy) * g[2] + wz;
+}
+
+// Render colour of a particle from its velocity: FluidSimCPU::updateColors
+// (fluidSimCPU.cc:100-125) over getSpeedNormalzied (physicsWorld.cc:178-182):
+// clamp(length(v), 0, 1.5) / 1.5 with glm's length = sqrt(dot(v, v)),
+// clamp = min(max(x, lo), hi), max(x, y) = x < y ? y : x, min(x, y) = y < x ?
+// y : x; then the 4-stop gradient with breakpoints 0.33f and 0.66f held in
+// float variables (so b2 - b1 and 1 - b2 are float operations), scalar * vec4
+// per component.  stops = Color1..Color4, RGBA each.
+SPH_HD void speed_color(float vx, float vy, float vz, const float *stops, float out[4])
+{
+    const float len = sqrtf((vx * vx + vy * vy) + vz * vz);
+    const float mx = len < 0.0f ? 0.0f : len;  // glm::max(len, 0)
+    const float nrm = (1.5f < mx ? 1.5f : mx) / 1.5f;
+    const float b1 = 0.33f, b2 = 0.66f;
+    const float *ca, *cb;
+    float t;
+    if (nrm <= b1) {
+        t = nrm / b1;
+        ca = stops;
+        cb = stops + 4;
+    } else if (nrm <= b2) {
+        t = (nrm - b1) / (b2 - b1);
+        ca = stops + 4;
+        cb = stops + 8;
+    } else {
+        t = (nrm - b2) / (1.0f - b2);
+        ca = stops + 8;
+        cb = stops + 12;
+    }
+    const float u = 1.0f - t;
+    for (int c = 0; c < 4; ++c) out[c] = u * ca[c] + t * cb[c];
 }
 
 // glm::sign for float (func_common.inl:144-150): (0 < x) - (x < 0).

@@ -241,6 +241,12 @@ int Engine::create(const std::vector<SceneSpec> &scenes, uint32_t capacity,
     e->ed_h_->n_total = p_off;
     e->ed_h_->table_total = k_off;
     e->ed_h_->n_scenes = (uint32_t)scenes.size();
+    {
+        // FluidSimCPU's gradient stops Color1..Color4 (fluidSimCPU.h:26-29)
+        const float stops[16] = {0.0f, 0.75f, 1.0f, 1.0f, 0.0f, 1.0f, 0.0f, 1.0f,
+                                 1.0f, 1.0f, 0.0f, 1.0f, 1.0f, 0.0f, 0.0f, 1.0f};
+        memcpy(e->ed_h_->stops, stops, sizeof(stops));
+    }
     e->key_bits_ = bits_for(table_capacity - 1);
     e->ed_h_->key_bits = e->key_bits_;
     work_key_layout(e->ed_h_->n_scenes, e->ed_h_);
@@ -339,6 +345,7 @@ Engine::~Engine()
         if (b) (void)hipFree(b);
     if (tmp_pos_) (void)hipFree(tmp_pos_);
     if (tmp_vel_) (void)hipFree(tmp_vel_);
+    if (colors_buf_) (void)hipFree(colors_buf_);
     if (ed_h_) (void)hipHostFree(ed_h_);
     if (sd_pinned_) (void)hipHostFree(sd_pinned_);
     for (hipEvent_t &e : ev_)
@@ -388,7 +395,8 @@ int Engine::repack(const std::vector<RepackItem> &items, std::string *err)
         }
         if (it.old_scene >= (int)sd_h_.size() ||
             (it.old_scene >= 0 && sd_h_[it.old_scene].n != it.spec.n) ||
-            (it.old_scene < 0 && (!it.pos4 || !it.vel4))) {
+            (it.old_scene < 0 && !it.gen && (!it.pos4 || !it.vel4)) ||
+            (it.gen && it.gen->n != it.spec.n)) {
             *err = "repack item does not match an existing scene or lacks its state";
             return SPH_ERR_INVALID;
         }
@@ -424,7 +432,9 @@ int Engine::repack(const std::vector<RepackItem> &items, std::string *err)
     for (size_t q = 0; q < items.size(); ++q) {
         const RepackItem &it = items[q];
         const uint32_t n = it.spec.n, o = new_off[q];
-        if (it.old_scene < 0) {
+        if (it.old_scene < 0 && it.gen) {
+            CK(launch_generate(*it.gen, d_.pos + o, d_.vel + o, stream_), "repack generate");
+        } else if (it.old_scene < 0) {
             CK(hipMemcpyAsync(d_.pos + o, it.pos4, f4 * n, hipMemcpyHostToDevice, stream_), "repack upload");
             CK(hipMemcpyAsync(d_.vel + o, it.vel4, f4 * n, hipMemcpyHostToDevice, stream_), "repack upload");
         } else if (sd_h_[it.old_scene].p_off != o) {
@@ -509,6 +519,56 @@ int Engine::upload(uint32_t s, const float *pos4, const float *vel4, std::string
         CK(hipMemcpyAsync(d_.vel + d.p_off, vel4, bytes, hipMemcpyHostToDevice, stream_),
            "upload velocities");
     CK(hipStreamSynchronize(stream_), "sync after upload");
+    return SPH_OK;
+}
+
+int Engine::set_colors(const float *stops, bool enable, std::string *err)
+{
+    if (stops) {
+        memcpy(ed_h_->stops, stops, sizeof(ed_h_->stops));
+        int rc = push_descs(err);
+        if (rc) return rc;
+    }
+    if (enable && !colors_buf_) {
+        CK(hipMalloc((void **)&colors_buf_, sizeof(float4) * (size_t)d_.capacity), "colour buffer");
+        device_bytes_ += sizeof(float4) * (size_t)d_.capacity;
+        CK(hipMemsetAsync(colors_buf_, 0, sizeof(float4) * (size_t)d_.capacity, stream_), "clear colours");
+        CK(hipStreamSynchronize(stream_), "sync colours");
+    }
+    float4 *want = enable ? colors_buf_ : nullptr;
+    if (want != d_.colors) {
+        d_.colors = want;
+        invalidate_graph();
+    }
+    return SPH_OK;
+}
+
+int Engine::download_colors(uint32_t s, float *rgba4, std::string *err)
+{
+    if (s >= sd_h_.size() || !rgba4) {
+        *err = "scene index out of range or null output";
+        return SPH_ERR_INVALID;
+    }
+    if (!colors_buf_) {
+        *err = "colours are not enabled";
+        return SPH_ERR_STATE;
+    }
+    const SceneDesc &d = sd_h_[s];
+    CK(hipMemcpyAsync(rgba4, colors_buf_ + d.p_off, sizeof(float4) * d.n, hipMemcpyDeviceToHost, stream_),
+       "download colours");
+    CK(hipStreamSynchronize(stream_), "sync colours");
+    return SPH_OK;
+}
+
+int Engine::generate(uint32_t s, const GenPlan &g, std::string *err)
+{
+    if (s >= sd_h_.size() || g.n != sd_h_[s].n) {
+        *err = "generate: scene index or particle count does not match";
+        return SPH_ERR_INVALID;
+    }
+    const SceneDesc &d = sd_h_[s];
+    CK(launch_generate(g, d_.pos + d.p_off, d_.vel + d.p_off, stream_), "generate");
+    CK(hipStreamSynchronize(stream_), "sync after generate");
     return SPH_OK;
 }
 

@@ -26,6 +26,7 @@ struct RepackItem {
     int old_scene;
     const float *pos4;
     const float *vel4;
+    const GenPlan *gen = nullptr;  // new scene generated on the device instead
 };
 
 // Fills the derived constants of a SceneDesc from user settings, with the
@@ -67,6 +68,13 @@ class Engine {
 
     int set_params(uint32_t s, const sph_params &p, std::string *err);
     int upload(uint32_t s, const float *pos4, const float *vel4, std::string *err);
+    // Initial state of scene s generated on the device (g.n must equal its count).
+    int generate(uint32_t s, const GenPlan &g, std::string *err);
+    // Render hand-off: gradient stops (16 floats, null keeps them) and
+    // whether each update writes the speed colours (id order, RGBA).
+    int set_colors(const float *stops, bool enable, std::string *err);
+    int download_colors(uint32_t s, float *rgba4, std::string *err);
+    const float4 *colors_buffer() const { return colors_buf_; }
     // One Update(dt) for every scene (physicsWorld.cc:39-111).
     int update(float dt, std::string *err);
     // `substeps` Update(dt / substeps) calls; blocks until done when sync.
@@ -152,6 +160,7 @@ class Engine {
     double stage_ms_[6] = {0, 0, 0, 0, 0, 0};
     uint64_t device_bytes_ = 0;
     float dt_cached_ = -1.0f;
+    float4 *colors_buf_ = nullptr;  // speed colours (allocated when first enabled)
     float4 *tmp_pos_ = nullptr;  // repack staging
     float4 *tmp_vel_ = nullptr;
     std::vector<hipEvent_t> prof_ev_;

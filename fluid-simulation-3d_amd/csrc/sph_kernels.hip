@@ -956,7 +956,8 @@ template <bool MULTI, bool WORK, bool LIST, bool NBR, bool GRID>
 __global__ __launch_bounds__(256) void k_viscosity_integrate(SPH_GATHER_PARAMS,
                                                               const float4 *__restrict__ vs2,
                                                               float4 *__restrict__ pos,
-                                                              float4 *__restrict__ vel)
+                                                              float4 *__restrict__ vel,
+                                                              float4 *__restrict__ colors)
 {
     SPH_GATHER_PROLOGUE();
     const float4 vi = vs2[k];
@@ -995,6 +996,12 @@ __global__ __launch_bounds__(256) void k_viscosity_integrate(SPH_GATHER_PARAMS,
     x.w = 0.34f;  // OutPositions[i] = vec4(positions[i], 0.34f), cc:107
     pos[g] = x;
     vel[g] = v;
+    if (colors) {
+        // render hand-off: the speed colour of the new velocity (fluidSimCPU.cc:100-125)
+        float c[4];
+        speed_color(v.x, v.y, v.z, ed->stops, c);
+        colors[g] = make_float4(c[0], c[1], c[2], c[3]);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1077,6 +1084,18 @@ __global__ void k_neighbour_fill(const EngineDesc *__restrict__ ed,
         uint64_t o = offs[g];
         const uint32_t p0 = S.p_off;
         walk_prefilter(S, xps, start, endt, xps[k], [&](uint32_t m) { list[o++] = svals[m] - p0; });
+    }
+}
+
+// Initial state generated on the device: the same per-particle function as
+// the host generator (sph_generate.h), so the bytes are identical.
+__global__ void k_generate(const GenPlan g, float4 *__restrict__ pos, float4 *__restrict__ vel)
+{
+    for (uint32_t q = global_id(); q < g.n; q += grid_stride()) {
+        float p[4], v[4];
+        generate_particle(g, q, p, v);
+        pos[q] = make_float4(p[0], p[1], p[2], p[3]);
+        vel[q] = make_float4(v[0], v[1], v[2], v[3]);
     }
 }
 
@@ -1244,7 +1263,7 @@ hipError_t launch_viscosity_integrate(const DevState &s, const LaunchCfg &c, int
                                       hipStream_t st)
 {
     SPH_GATHER_LAUNCH(k_viscosity_integrate, c.viscosity, c.viscosity.block, st, SPH_GATHER_ARGS,
-                      s.vs2, s.pos, s.vel);
+                      s.vs2, s.pos, s.vel, s.colors);
     SPH_LAUNCH_CHECK();
 }
 
@@ -1276,6 +1295,12 @@ hipError_t launch_neighbour_fill(const DevState &s, int buf, const uint64_t *off
     else
         k_neighbour_fill<false><<<g, 256, 0, st>>>(s.ed, s.sd, s.scene_of, s.xps, s.start, s.end,
                                                    s.vals[buf], offs, list);
+    SPH_LAUNCH_CHECK();
+}
+
+hipError_t launch_generate(const GenPlan &g, float4 *pos, float4 *vel, hipStream_t st)
+{
+    k_generate<<<grid_for(g.n, 256), 256, 0, st>>>(g, pos, vel);
     SPH_LAUNCH_CHECK();
 }
 

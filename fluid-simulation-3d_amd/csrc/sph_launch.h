@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "sph_generate.h"
 #include "sph_types.h"
 
 namespace sph {
@@ -37,6 +38,7 @@ struct DevState {
     uint32_t *wvals[2];  // particle ids (sort buffers); the sorted ones are the work order
     const uint32_t *work;  // = wvals[result] of the last rebuild
     uint32_t *rank;      // [capacity] sorted slot of each particle id
+    float4 *colors;      // [capacity] speed colours, id order (null: not written)
     uint32_t *nbr;       // [capacity * 64] neighbour rows by work index (nbr mode)
     uint32_t *ncnt;      // [capacity] neighbour count per row
     uint32_t capacity;       // particle slots allocated
@@ -106,6 +108,9 @@ hipError_t launch_unpermute_dens(const DevState &s, int buf, float2 *out, hipStr
 hipError_t launch_neighbour_count(const DevState &s, int buf, uint32_t *counts, hipStream_t st);
 hipError_t launch_neighbour_fill(const DevState &s, int buf, const uint64_t *offs,
                                  uint32_t *list, hipStream_t st);
+// Initial state of one scene from its plan (sph_generate.h), n = g.n slots
+// starting at pos / vel.
+hipError_t launch_generate(const GenPlan &g, float4 *pos, float4 *vel, hipStream_t st);
 // Plain copy of n float4 (HBM counter calibration).
 hipError_t launch_copy(const float4 *src, float4 *dst, size_t n, hipStream_t st);
 // Count of non-finite position/velocity components (error flag).

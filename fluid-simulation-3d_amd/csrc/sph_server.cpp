@@ -51,6 +51,8 @@ struct Request {
     sph_params params{};
     uint32_t n = 0, table = 0;
     std::vector<float> pos, vel, dens;  // input until admitted, results once done
+    bool generated = false;              // initial state generated on the device at admission
+    sph::GenPlan gen{};
     uint64_t updates_total = 0, updates_done = 0;
     int32_t state = SPH_REQ_QUEUED;
     double t_submit = 0, t_admit = 0, t_done = 0;
@@ -168,7 +170,8 @@ int repack(sph_server_t *s, const std::vector<uint64_t> &admit, std::string *err
     }
     for (uint64_t id : admit) {
         Request &r = s->reqs.at(id);
-        items.push_back({{r.params, r.n, r.table}, -1, r.pos.data(), r.vel.data()});
+        items.push_back({{r.params, r.n, r.table}, -1, r.pos.data(), r.vel.data(),
+                         r.generated ? &r.gen : nullptr});
         order.push_back(id);
         need_n += r.n;
         need_t += r.table;
@@ -290,6 +293,37 @@ int sph_server_submit(sph_server s, const sph_params *p, uint32_t n, uint32_t ta
     }
     r.pos.assign(pos4, pos4 + 4 * (size_t)n);
     r.vel.assign(vel4, vel4 + 4 * (size_t)n);
+    r.updates_total = (uint64_t)steps * (uint64_t)s->cfg.substeps;
+    r.t_submit = now_s();
+    *id = r.id;
+    s->queue.push_back(r.id);
+    s->reqs.emplace(r.id, std::move(r));
+    return SPH_OK;
+    SRV_GUARD_END
+}
+
+int sph_server_submit_generated(sph_server s, const sph_params *p, uint32_t n,
+                                uint32_t table_size, int kind, uint64_t seed, const float box[3],
+                                float spacing, float jitter, float speed, uint32_t steps,
+                                uint64_t *id)
+{
+    SRV_GUARD_BEGIN
+    if (!s || !p || !id) return fail(SPH_ERR_INVALID, "null argument");
+    if (n == 0 || steps == 0) return fail(SPH_ERR_INVALID, "n and steps must be > 0");
+    if (const char *m = sph::check_params(*p)) return fail(SPH_ERR_INVALID, m);
+    Request r;
+    if (sph::plan_generate(kind, seed, n, box, spacing, jitter, speed, &r.gen))
+        return fail(SPH_ERR_INVALID, "invalid scene generator arguments");
+    r.generated = true;
+    r.id = s->next_id++;
+    r.params = *p;
+    r.n = n;
+    r.table = table_size ? table_size : sph::grid_table_size(*p, n);
+    r.bytes = request_bytes(r.n, r.table);
+    if (r.bytes > s->cfg.hbm_budget_bytes) {
+        ++s->rejected;
+        return fail(SPH_ERR_BUDGET, "request exceeds the HBM budget on its own");
+    }
     r.updates_total = (uint64_t)steps * (uint64_t)s->cfg.substeps;
     r.t_submit = now_s();
     *id = r.id;

@@ -129,6 +129,12 @@ def lib():
         "sph_get_size": (i32, [H, p, p]),
         "sph_upload": (i32, [H, p, p]),
         "sph_initialize_data": (i32, [H]),
+        "sph_initialize": (i32, [H, i32, u64, p, f, f, f]),
+        "sph_set_colors": (i32, [H, p, i32]),
+        "sph_download_colors": (i32, [H, p]),
+        "sph_render_buffers": (i32, [H, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p)]),
+        "sph_batch_initialize": (i32, [H, u32, i32, u64, p, f, f, f]),
+        "sph_server_submit_generated": (i32, [H, pp, u32, u32, i32, u64, p, f, f, f, u32, p]),
         "sph_update": (i32, [H, f]),
         "sph_step": (i32, [H, f, i32]),
         "sph_step_async": (i32, [H, f, i32]),
@@ -306,6 +312,28 @@ class Sim:
     def initialize_data(self):
         _check(lib().sph_initialize_data(self._h))
 
+    def initialize(self, kind, seed=0, box=(1.0, 1.0, 1.0), spacing=0.0, jitter=0.0, speed=0.0):
+        """Initial state generated on the device (bytes of generate() for the
+        same arguments), then the spatial lookup and densities."""
+        b = np.asarray(box, np.float32)
+        _check(lib().sph_initialize(self._h, int(kind), int(seed), _ptr(b), spacing, jitter, speed))
+
+    def set_colors(self, stops=None, enable=True):
+        """Render hand-off: speed colours written by every update."""
+        st = None if stops is None else np.ascontiguousarray(stops, np.float32).reshape(16)
+        _check(lib().sph_set_colors(self._h, _ptr(st), 1 if enable else 0))
+
+    def download_colors(self):
+        c = np.zeros((self.n, 4), np.float32)
+        _check(lib().sph_download_colors(self._h, _ptr(c)))
+        return c
+
+    def render_buffers(self):
+        """(positions, colors) device pointers (SSBO bindings 0 and 1); colors 0 if off."""
+        pp_, cp_ = ctypes.c_void_p(), ctypes.c_void_p()
+        _check(lib().sph_render_buffers(self._h, ctypes.byref(pp_), ctypes.byref(cp_)))
+        return pp_.value or 0, cp_.value or 0
+
     def update(self, dt):
         _check(lib().sph_update(self._h, dt))
 
@@ -435,6 +463,12 @@ class Batch:
                 raise ValueError(f"expected shape {(n, 4)}, got {a.shape}")
         _check(lib().sph_batch_upload(self._h, scene, _ptr(pos), _ptr(vel)))
 
+    def initialize(self, scene, kind, seed=0, box=(1.0, 1.0, 1.0), spacing=0.0, jitter=0.0,
+                   speed=0.0):
+        b = np.asarray(box, np.float32)
+        _check(lib().sph_batch_initialize(self._h, scene, int(kind), int(seed), _ptr(b), spacing,
+                                          jitter, speed))
+
     def step(self, dt, substeps=1):
         _check(lib().sph_batch_step(self._h, dt, substeps))
 
@@ -510,6 +544,16 @@ class Server:
             self.close()
         except Exception:
             pass
+
+    def submit_generated(self, params, n, steps, kind, seed=0, box=(1.0, 1.0, 1.0), spacing=0.0,
+                         jitter=0.0, speed=0.0, table=0):
+        """A request whose initial state is generated on the device at admission."""
+        b = np.asarray(box, np.float32)
+        rid = np.zeros(1, np.uint64)
+        _check(lib().sph_server_submit_generated(self._h, ctypes.byref(params), int(n), int(table),
+                                                 int(kind), int(seed), _ptr(b), spacing, jitter,
+                                                 speed, int(steps), _ptr(rid)))
+        return int(rid[0])
 
     def submit(self, params, pos4, vel4, steps, table=0):
         pos = np.ascontiguousarray(pos4, np.float32)

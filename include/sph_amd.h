@@ -119,6 +119,31 @@ int sph_upload(sph_handle h, const float *pos4, const float *vel4);
  * (cc:518-557), zero velocity, then the spatial lookup and densities of that
  * state (cc:144-145). */
 int sph_initialize_data(sph_handle h);
+/* InitializeData for any synthetic scene (SPH_INIT_*, below), generated on
+ * the device -- no host arrays cross PCIe -- with exactly the bytes
+ * sph_generate writes for the same arguments; then the spatial lookup and
+ * densities of that state (cc:144-145).  sph_initialize_data is this with
+ * SPH_INIT_GRID_ARRANGEMENT (the reference's lattice, cc:518-557). */
+int sph_initialize(sph_handle h, int kind, uint64_t seed, const float box[3], float spacing,
+                   float jitter, float speed);
+/* Render hand-off (FluidSimCPU::updateColors, fluidSimCPU.cc:100-125; the
+ * GLSL path's UpdateColors, compute_particle.glsl:279-304): when enabled,
+ * every update also writes each particle's speed colour -- the 4-stop
+ * gradient of clamp(|v|, 0, 1.5) / 1.5 (getSpeedNormalzied,
+ * physicsWorld.cc:178-182) -- fused into the integration kernel.  stops:
+ * Color1..Color4 as RGBA (16 floats; NULL keeps the current ones, initially
+ * fluidSimCPU.h:26-29). */
+int sph_set_colors(sph_handle h, const float stops[16], int enable);
+/* The colours of the last update, n float4 RGBA in particle order. */
+int sph_download_colors(sph_handle h, float *rgba4);
+/* Device pointers of the render buffers, laid out as the reference's shader
+ * storage blocks (std430, fluidSimGPU.cc:98-104; particleRender.vs reads
+ * bindings 0 and 1): positions = binding 0 ReadPosAndScale (vec4 x, y, z,
+ * 0.34 -- OutPositions), colors = binding 1 ReadColors (vec4 RGBA; NULL
+ * until sph_set_colors enables them).  n entries each, particle order,
+ * valid until sph_destroy and rewritten by every update, so a renderer can
+ * copy them into (or map them as) its SSBOs with no host round trip. */
+int sph_render_buffers(sph_handle h, const void **positions, const void **colors);
 /* Update(deltatime) (cc:39-111), blocking. */
 int sph_update(sph_handle h, float dt);
 /* `substeps` calls of Update(dt / substeps); blocking. */
@@ -204,6 +229,10 @@ int sph_batch_create(uint32_t n_scenes, const uint32_t *counts, const uint32_t *
                      const sph_params *params, sph_batch *out);
 int sph_batch_destroy(sph_batch b);
 int sph_batch_upload(sph_batch b, uint32_t scene, const float *pos4, const float *vel4);
+/* Scene `scene`'s initial state generated on the device (as sph_initialize,
+ * without the density pass; the next step computes it). */
+int sph_batch_initialize(sph_batch b, uint32_t scene, int kind, uint64_t seed, const float box[3],
+                         float spacing, float jitter, float speed);
 int sph_batch_set_params(sph_batch b, uint32_t scene, const sph_params *p);
 /* Every scene: `substeps` Update(dt / substeps) calls; blocking / async. */
 int sph_batch_step(sph_batch b, float dt, int substeps);
@@ -267,6 +296,13 @@ int sph_server_create(const sph_server_config *cfg, sph_server *out);
 int sph_server_destroy(sph_server s);
 int sph_server_submit(sph_server s, const sph_params *p, uint32_t n, uint32_t table_size,
                       const float *pos4, const float *vel4, uint32_t steps, uint64_t *id);
+/* A bake request whose initial state is generated on the device at admission
+ * (SPH_INIT_* arguments as sph_generate, identical bytes): nothing is held
+ * in host memory while it waits. */
+int sph_server_submit_generated(sph_server s, const sph_params *p, uint32_t n,
+                                uint32_t table_size, int kind, uint64_t seed, const float box[3],
+                                float spacing, float jitter, float speed, uint32_t steps,
+                                uint64_t *id);
 int sph_server_tick(sph_server s, sph_server_tick_info *info);
 int sph_server_status(sph_server s, uint64_t id, int32_t *state, uint64_t *updates_done,
                       double *latency_ms);
@@ -276,7 +312,9 @@ int sph_server_fetch(sph_server s, uint64_t id, float *pos4, float *vel4, float 
 int sph_server_cancel(sph_server s, uint64_t id);
 int sph_server_get_stats(sph_server s, sph_server_stats *st);
 
-/* ---- synthetic scenes (host only; identical bytes for every consumer) ---- */
+/* ---- synthetic scenes: sph_generate writes host arrays; sph_initialize,
+ * sph_batch_initialize and sph_server_submit_generated generate the same
+ * bytes on the device (one per-particle function for both) ---- */
 #define SPH_INIT_GRID_ARRANGEMENT 0 /* the reference's InitializeData lattice, cc:518-557 */
 #define SPH_INIT_DAM_BREAK 1        /* block at the -x,-y,-z corner, dims nx:2nx:2nx, z fastest */
 #define SPH_INIT_UNIFORM_LOWER 2    /* uniform in the lower half of the box */

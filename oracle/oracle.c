@@ -557,3 +557,35 @@ void orc_grid_arrangement(uint32_t n, float *pos4)
                 i++;
             }
 }
+
+/* ------------------------------------------------------------------ */
+/* render hand-off, fluidSimCPU.cc:100-125 + physicsWorld.cc:178-182  */
+/* ------------------------------------------------------------------ */
+static inline float glm_min(float x, float y) { return (y < x) ? y : x; } /* func_common.inl:17-21 */
+static inline float glm_max(float x, float y) { return (x < y) ? y : x; } /* func_common.inl:25-30 */
+
+void orc_speed_colors(uint32_t n, const float *vel4, const float stops[16], float *rgba4)
+{
+    const float *Color1 = stops, *Color2 = stops + 4, *Color3 = stops + 8, *Color4 = stops + 12;
+    for (uint32_t i = 0; i < n; ++i) {
+        const float *v = &vel4[4 * (size_t)i];
+        /* getSpeedNormalzied: glm::clamp(glm::length(velocity), 0.0f, 1.5f) / 1.5f */
+        float len = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        float normalized = glm_min(glm_max(len, 0.0f), 1.5f) / 1.5f;
+        float breakpoint1 = 0.33f; /* fluidSimCPU.cc:107-108 */
+        float breakpoint2 = 0.66f;
+        float *out = &rgba4[4 * (size_t)i];
+        for (int c = 0; c < 4; ++c) {
+            if (normalized <= breakpoint1) {
+                out[c] = (1.0f - normalized / breakpoint1) * Color1[c] +
+                         (normalized / breakpoint1) * Color2[c];
+            } else if (normalized <= breakpoint2) {
+                out[c] = (1.0f - (normalized - breakpoint1) / (breakpoint2 - breakpoint1)) * Color2[c] +
+                         ((normalized - breakpoint1) / (breakpoint2 - breakpoint1)) * Color3[c];
+            } else {
+                out[c] = (1.0f - (normalized - breakpoint2) / (1.0f - breakpoint2)) * Color3[c] +
+                         ((normalized - breakpoint2) / (1.0f - breakpoint2)) * Color4[c];
+            }
+        }
+    }
+}

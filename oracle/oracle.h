@@ -155,6 +155,14 @@ void orc64_viscosity(uint32_t n, uint32_t table_size, const orc_params *p, doubl
                      const uint32_t *start, const double *vin3,
                      double *dv3, double *mag3);
 
+/* ---- render hand-off: FluidSimCPU::updateColors (fluidSimCPU.cc:100-125)
+ * over getSpeedNormalzied (physicsWorld.cc:178-182).  rgba4[4i..4i+3] = the
+ * colour of particle i from its velocity vel4[4i..4i+2]; stops = Color1..4
+ * RGBA (fluidSimCPU.h:26-29 by default).  glm semantics: length =
+ * sqrt(dot(v, v)), clamp = min(max(x, lo), hi) with max(x, y) = x < y ? y : x
+ * and min(x, y) = y < x ? y : x (func_common.inl:17-30, 504-508). ---- */
+void orc_speed_colors(uint32_t n, const float *vel4, const float stops[16], float *rgba4);
+
 /* ---- GridArrangement (cc:112-147, 518-557): the reference's initial lattice ---- */
 void orc_grid_arrangement(uint32_t n, float *pos4);
 

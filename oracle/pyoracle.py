@@ -70,6 +70,7 @@ def lib():
             "orc64_pressure": (None, [u32, u32, pp, ctypes.c_double, p, p, p, p, p, p, p, p]),
             "orc64_viscosity": (None, [u32, u32, pp, ctypes.c_double, p, p, p, p, p, p, p, p]),
             "orc_grid_arrangement": (None, [u32, p]),
+            "orc_speed_colors": (None, [u32, p, p, p]),
             "orc_set_threads": (None, [i32]),
             "orc_get_threads": (i32, []),
         }
@@ -274,3 +275,16 @@ def grid_arrangement(n):
     pos = np.zeros((n, 4), np.float32)
     lib().orc_grid_arrangement(n, _ptr(pos))
     return pos
+
+
+REFERENCE_COLOR_STOPS = np.array([0.0, 0.75, 1.0, 1.0, 0.0, 1.0, 0.0, 1.0,
+                                  1.0, 1.0, 0.0, 1.0, 1.0, 0.0, 0.0, 1.0], np.float32)
+
+
+def speed_colors(vel4, stops=None):
+    """FluidSimCPU::updateColors (fluidSimCPU.cc:100-125): RGBA per particle."""
+    v = np.ascontiguousarray(vel4, np.float32)
+    st = REFERENCE_COLOR_STOPS if stops is None else np.ascontiguousarray(stops, np.float32)
+    out = np.zeros((v.shape[0], 4), np.float32)
+    lib().orc_speed_colors(v.shape[0], _ptr(v), _ptr(st), _ptr(out))
+    return out
