@@ -193,22 +193,48 @@ namespace sph_amd {
 // application's FluidSimBase and forwards to these.
 class FluidSimHIPCore {
   public:
+    using vec4 = Physics::Fluid::FluidSimulation::vec4;
     void initialize(int particleAmount)  // fluidSimCPU.cc:9-33
     {
         n_ = particleAmount;
         Physics::Fluid::FluidSimulation::getInstance().InitializeData(particleAmount);
+        if (colors_on_) set_colors(stops_);
     }
     void update(float dt) { Physics::Fluid::FluidSimulation::getInstance().Update(dt); }  // :35-40
-    void reset() { Physics::Fluid::FluidSimulation::getInstance().InitializeData(n_); }   // :42-46
+    void reset() { initialize(n_); }                                                      // :42-46
     void cleanup() {}                                                                     // :48-51
     // the SSBO upload source of render(): OutPositions (fluidSimCPU.cc:104)
-    const std::vector<Physics::Fluid::FluidSimulation::vec4> &out_positions() const
+    const std::vector<vec4> &out_positions() const
     {
         return Physics::Fluid::FluidSimulation::getInstance().OutPositions;
+    }
+    // FluidSimCPU::updateColors (fluidSimCPU.cc:100-125), computed on the GPU
+    // inside every Update from then on: stops = Color1..Color4 (RGBA each).
+    void set_colors(const float stops[16])
+    {
+        for (int c = 0; c < 16; ++c) stops_[c] = stops[c];
+        colors_on_ = true;
+        sph_handle h = Physics::Fluid::FluidSimulation::getInstance().handle();
+        if (h && sph_set_colors(h, stops_, 1) != SPH_OK)
+            throw std::runtime_error(std::string("sph_amd: ") + sph_last_error());
+    }
+    // The colours of the last Update (the render() colour buffer, fluidSimCPU.cc:80-83).
+    const std::vector<vec4> &colors()
+    {
+        sph_handle h = Physics::Fluid::FluidSimulation::getInstance().handle();
+        colors_.resize((size_t)n_);
+        if (!h || !colors_on_ || n_ <= 0) return colors_;
+        if (sph_download_colors(h, reinterpret_cast<float *>(colors_.data())) != SPH_OK)
+            throw std::runtime_error(std::string("sph_amd: ") + sph_last_error());
+        return colors_;
     }
 
   private:
     int n_ = 0;
+    bool colors_on_ = false;
+    float stops_[16] = {0.0f, 0.75f, 1.0f, 1.0f, 0.0f, 1.0f, 0.0f, 1.0f,
+                        1.0f, 1.0f, 0.0f, 1.0f, 1.0f, 0.0f, 0.0f, 1.0f};  // fluidSimCPU.h:26-29
+    std::vector<vec4> colors_;
 };
 
 }  // namespace sph_amd

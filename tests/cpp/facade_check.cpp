@@ -1,8 +1,9 @@
 // Drives the C++ drop-in (include/sph_fluid_simulation.hpp) the way the
 // reference application drives Physics::Fluid::FluidSimulation
 // (gameApp.cc:112-113, 231, 363-411): InitializeData, settings, Update per
-// frame, getters.  Writes the final positions, velocities and densities
-// (float32) to argv[1] for the GPU test to compare with the C-ABI path.
+// frame, getters; and the FluidSimBase-shaped backend with render colours.
+// Writes the final positions, velocities, densities and colours (float32) to
+// argv[1] for the GPU test to compare with the C-ABI path and the oracle.
 #include <stdio.h>
 
 #include "sph_fluid_simulation.hpp"
@@ -14,7 +15,13 @@ int main(int argc, char **argv)
         return 2;
     }
     auto &sim = Physics::Fluid::FluidSimulation::getInstance();
-    sim.InitializeData(10000);  // particleAmount, gameApp.cc:41
+    // the FluidSimBase-shaped backend: initialize = InitializeData(particleAmount)
+    // (fluidSimCPU.cc:9-33, gameApp.cc:41), colours on with FluidSimCPU's stops
+    sph_amd::FluidSimHIPCore core;
+    core.initialize(10000);
+    const float stops[16] = {0.0f, 0.75f, 1.0f, 1.0f, 0.0f, 1.0f, 0.0f, 1.0f,
+                             1.0f, 1.0f, 0.0f, 1.0f, 1.0f, 0.0f, 0.0f, 1.0f};
+    core.set_colors(stops);
     sim.setGravity(true);
     sim.setGravityScale(9.81f);
     sim.setViscosityStrength(0.3f);
@@ -24,12 +31,15 @@ int main(int argc, char **argv)
     if (sim.getDensity(10000) != 0.0f || sim.getPosition(99999).x != 0.0f) return 3;
     if (sim.positions.size() != 10000 || sim.OutPositions[5].w != 0.34f) return 4;
     if (!(sim.getElapsedTimeDensity() > 0.0)) return 5;
+    const auto &col = core.colors();
+    if (col.size() != 10000 || core.out_positions().size() != 10000) return 7;
     FILE *f = fopen(argv[1], "wb");
     if (!f) return 6;
     for (uint32_t i = 0; i < 10000; ++i) {
         const auto p = sim.getPosition(i), v = sim.getVelocity(i);
-        const float rec[8] = {p.x, p.y, p.z, v.x, v.y, v.z, sim.getDensity(i), sim.getNearDensity(i)};
-        fwrite(rec, sizeof(float), 8, f);
+        const float rec[12] = {p.x, p.y, p.z, v.x, v.y, v.z, sim.getDensity(i), sim.getNearDensity(i),
+                               col[i].x, col[i].y, col[i].z, col[i].w};
+        fwrite(rec, sizeof(float), 12, f);
     }
     fclose(f);
     printf("facade ok: speed[0]=%g normalized=%g\n", sim.getSpeed(0), sim.getSpeedNormalzied(0));
