@@ -86,6 +86,16 @@ const char *sph_version(void) { return "sph_amd 0.1 (gfx950)"; }
 
 uint32_t sph_hash_cell(int32_t cx, int32_t cy, int32_t cz) { return sph::hash_cell(cx, cy, cz); }
 
+int sph_debug_quotients(const float *a, const float *b, float *out, uint64_t n)
+{
+    if (n && (!a || !b || !out)) return fail(SPH_ERR_INVALID, "null argument");
+    for (uint64_t i = 0; i < n; ++i) {
+        const float y = sph::reciprocal_or_zero(b[i]);
+        out[i] = (y != 0.0f && sph::quotient_ok(a[i])) ? sph::quotient(a[i], b[i], y) : a[i] / b[i];
+    }
+    return SPH_OK;
+}
+
 uint32_t sph_bucket_of(uint32_t hash, uint32_t table_size)
 {
     if (table_size == 0) return 0;

@@ -84,6 +84,33 @@ SPH_HD void speed_color(float vx, float vy, float vz, const float *stops, float 
     for (int c = 0; c < 4; ++c) out[c] = u * ca[c] + t * cb[c];
 }
 
+// Correctly rounded quotient a / b from y = RN(1/b), without a divide
+// (Markstein's theorem: with y within half an ulp of 1/b and q0 = RN(a*y)
+// within one ulp of a/b, r = a - b*q0 is exact and RN(q0 + r*y) = RN(a/b),
+// provided nothing under- or overflows).  quotient_ok() states the range in
+// which that holds for the kernels' use: b in [2^-26, 2^26] (y stored only
+// then), and a = 0 or |a| in [2^-91, 2^100] -- so r is exact (|a| >= 2^-103),
+// y and q0 are normal and q0 is finite.  (For a = -0 the result is +0 where
+// IEEE gives -0; a zero term does not change a sum that starts at +0.)
+SPH_HD bool quotient_ok(float a)
+{
+    const float m = fabsf(a);
+    return a == 0.0f || (m >= 0x1p-91f && m <= 0x1p100f);
+}
+
+SPH_HD float quotient(float a, float b, float y)
+{
+    const float q0 = a * y;
+    const float r = fmaf(-b, q0, a);
+    return fmaf(r, y, q0);
+}
+
+// y = RN(1/x) for x in [2^-26, 2^26], else 0 (no fast quotient by x).
+SPH_HD float reciprocal_or_zero(float x)
+{
+    return (x >= 0x1p-26f && x <= 0x1p26f) ? 1.0f / x : 0.0f;
+}
+
 // glm::sign for float (func_common.inl:144-150): (0 < x) - (x < 0).
 SPH_HD float sign_f(float x) { return (float)(0.0f < x) - (float)(x < 0.0f); }
 

@@ -854,7 +854,8 @@ __global__ __launch_bounds__(256) void k_density(SPH_GATHER_PARAMS, float4 *__re
                                                 MaskOut{nbr, ncnt, ed->n_total});
     if (ROWS) ncnt[i] = c;
     // pressures of cc:371-372 / 407-408, computed once per particle
-    dens[k] = make_float4(rho, rhon, (rho - S.rho0) * S.k, rhon * S.k_near);
+    // the pressure pass divides by these densities: their reciprocals ride along
+    dens[k] = make_float4(rho, rhon, reciprocal_or_zero(rho), reciprocal_or_zero(rhon));
 }
 
 // Density pass of the neighbour-row mode.  Each work-item stages its row in
@@ -891,7 +892,7 @@ __global__ __launch_bounds__(256) void k_density_rows(SPH_GATHER_PARAMS, float4 
                 ++c;
             }
         });
-        dens[k] = make_float4(rho, rhon, (rho - S.rho0) * S.k, rhon * S.k_near);
+        dens[k] = make_float4(rho, rhon, reciprocal_or_zero(rho), reciprocal_or_zero(rhon));
         ncnt[i] = c;
     }
     cnt[t] = c;
@@ -913,12 +914,16 @@ __global__ __launch_bounds__(256) void k_pressure(SPH_GATHER_PARAMS, const float
 {
     SPH_GATHER_PROLOGUE();
     const float4 di = dens[k];
-    const float h = S.h, dw2c = S.dw2c, dw3c = S.dw3c;
-    const float P = di.z, Pn = di.w;
+    const float h = S.h, dw2c = S.dw2c, dw3c = S.dw3c, rho0 = S.rho0, kp = S.k, kn = S.k_near;
+    const float P = (di.x - rho0) * kp, Pn = di.y * kn;  // cc:371-372
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    // dj = (rho_j, rho_near_j, RN(1/rho_j), RN(1/rho_near_j)); the six
+    // divisions by rho_j / rho_near_j use those reciprocals (quotient(), exact
+    // in its stated range) and fall back to IEEE division outside it
     auto body = [&](uint32_t, float ox, float oy, float oz, float d2, const float4 &dj) {
-        const float sp = (P + dj.z) * 0.5f;   // cc:410
-        const float snp = (Pn + dj.w) * 0.5f; // cc:411
+        const float pj = (dj.x - rho0) * kp, pnj = dj.y * kn;  // cc:407-408
+        const float sp = (P + pj) * 0.5f;     // cc:410
+        const float snp = (Pn + pnj) * 0.5f;  // cc:411
         const float d = sqrtf(d2);
         float dirx = 0.0f, diry = 1.0f, dirz = 0.0f;  // cc:414
         if (d > 0.0f) {
@@ -928,13 +933,27 @@ __global__ __launch_bounds__(256) void k_pressure(SPH_GATHER_PARAMS, const float
         }
         const float v = h - d;
         const float a = -v * dw2c;      // SmoothingDerivativePow2, kernels.h:55
-        fx += dirx * a * sp / dj.x;     // cc:416
-        fy += diry * a * sp / dj.x;
-        fz += dirz * a * sp / dj.x;
         const float b = -v * v * dw3c;  // SmoothingDerivativePow3, kernels.h:67
-        fx += dirx * b * snp / dj.y;    // cc:417
-        fy += diry * b * snp / dj.y;
-        fz += dirz * b * snp / dj.y;
+        const float tx = dirx * a * sp, ty = diry * a * sp, tz = dirz * a * sp;
+        const float ux = dirx * b * snp, uy = diry * b * snp, uz = dirz * b * snp;
+        const bool fast = ((int)(dj.z != 0.0f) & (int)(dj.w != 0.0f) & (int)quotient_ok(tx) &
+                           (int)quotient_ok(ty) & (int)quotient_ok(tz) & (int)quotient_ok(ux) &
+                           (int)quotient_ok(uy) & (int)quotient_ok(uz)) != 0;
+        if (fast) {
+            fx += quotient(tx, dj.x, dj.z);  // cc:416
+            fy += quotient(ty, dj.x, dj.z);
+            fz += quotient(tz, dj.x, dj.z);
+            fx += quotient(ux, dj.y, dj.w);  // cc:417
+            fy += quotient(uy, dj.y, dj.w);
+            fz += quotient(uz, dj.y, dj.w);
+        } else {
+            fx += tx / dj.x;
+            fy += ty / dj.x;
+            fz += tz / dj.x;
+            fx += ux / dj.y;
+            fy += uy / dj.y;
+            fz += uz / dj.y;
+        }
     };
     if (NBR && GRID) {
         if (ncnt[k]) gather_masks<KB, true, true>(xps, dens, nbr, ed->n_total, k, pi, mtab, body);

@@ -30,7 +30,7 @@ struct DevState {
     float4 *xps;   // predicted position, w = float(hash) (the reference record's .y)
     float4 *vs;    // velocity after stage A, sorted
     float4 *vs2;   // velocity after the pressure stage, sorted
-    float4 *dens;  // rho, rho_near, pressure, near pressure
+    float4 *dens;  // rho, rho_near, RN(1/rho), RN(1/rho_near) (reciprocals 0 out of range)
     uint32_t *start;  // [table_total] first sorted slot of each bucket
     uint32_t *end;    // [table_total] one past the last slot (valid where start is)
     // work order of the gather kernels

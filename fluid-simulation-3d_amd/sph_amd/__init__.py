@@ -179,6 +179,7 @@ def lib():
         "sph_copy_calibration": (i32, [u64, i32, p]),
         "sph_hash_cell": (u32, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
         "sph_bucket_of": (u32, [u32, u32]),
+        "sph_debug_quotients": (i32, [p, p, p, u64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -235,6 +236,15 @@ def copy_calibration(nbytes, reps=3):
 def hash_cell(cx, cy, cz):
     """HashCell (physicsWorld.cc:505-511) as the kernels compute it."""
     return lib().sph_hash_cell(int(cx), int(cy), int(cz))
+
+
+def debug_quotients(a, b):
+    """a / b as the pressure kernel computes it (host build of the same code)."""
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    out = np.empty_like(a)
+    _check(lib().sph_debug_quotients(_ptr(a), _ptr(b), _ptr(out), a.size))
+    return out
 
 
 def bucket_of(hash_, table):

@@ -212,6 +212,11 @@ int sph_debug_neighbours(sph_handle h, uint64_t *offs, uint32_t *list, uint64_t 
  * GetKeyFromHash (cc:513-516), i.e. hash % table_size. ---- */
 uint32_t sph_hash_cell(int32_t cx, int32_t cy, int32_t cz);
 uint32_t sph_bucket_of(uint32_t hash, uint32_t table_size);
+/* Parity support, no reference counterpart: out[i] = a[i] / b[i] as the
+ * pressure kernel computes it (the reciprocal-and-correction quotient where
+ * its range conditions hold, IEEE division elsewhere), on the host from the
+ * same source.  Equal to IEEE a / b except that -0 / b gives +0. */
+int sph_debug_quotients(const float *a, const float *b, float *out, uint64_t n);
 
 /* ---- batching: many independent scenes per launch ----
  * No reference counterpart (the reference simulates one scene per process,

@@ -87,3 +87,36 @@ def test_null_handle_is_rejected():
     lib = S.lib()
     assert lib.sph_update(None, 0.01) == S.SPH_ERR_INVALID
     assert lib.sph_destroy(None) == S.SPH_OK
+
+
+def test_kernel_quotient_equals_ieee_division():
+    """The pressure kernel's quotient (reciprocal RN(1/b) with one correction,
+    Markstein) against IEEE float32 division: every significand of b in [1, 2)
+    with fixed numerators, every significand of a with sampled b, scaled
+    operands, zeros, and the out-of-range fallbacks."""
+    rng = np.random.default_rng(7)
+    mb = np.arange(1 << 23, dtype=np.uint32)
+    b_all = (np.uint32(0x3F800000) | mb).view(np.float32)
+    for a0 in (1.0, 1.9999999, 1.5, -1.25, 1.0000001):
+        a = np.full(b_all.shape, a0, np.float32)
+        got = S.debug_quotients(a, b_all)
+        want = (a / b_all).astype(np.float32)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), a0
+    a_all = (np.uint32(0x3F800000) | mb).view(np.float32)
+    for b0 in rng.uniform(1.0, 2.0, 4).astype(np.float32):
+        b = np.full(a_all.shape, b0, np.float32)
+        got = S.debug_quotients(a_all, b)
+        assert np.array_equal(got.view(np.uint32), (a_all / b).view(np.uint32)), b0
+    # scaled operands across the whole range, including outside the fast range
+    e_a = rng.integers(-140, 127, 1 << 20)
+    e_b = rng.integers(-30, 30, 1 << 20)
+    a = (rng.uniform(1, 2, 1 << 20) * np.exp2(e_a.astype(np.float64))).astype(np.float32)
+    a *= np.where(rng.random(1 << 20) < 0.5, -1, 1).astype(np.float32)
+    b = (rng.uniform(1, 2, 1 << 20) * np.exp2(e_b.astype(np.float64))).astype(np.float32)
+    with np.errstate(over="ignore", under="ignore"):
+        want = (a / b).astype(np.float32)
+    got = S.debug_quotients(a, b)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    # zeros: +0 exact; -0 gives +0 (a zero term leaves a sum from +0 unchanged)
+    z = S.debug_quotients(np.array([0.0, -0.0], np.float32), np.array([3.0, 3.0], np.float32))
+    assert z[0] == 0 and not np.signbit(z[0]) and z[1] == 0
