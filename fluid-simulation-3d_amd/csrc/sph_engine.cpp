@@ -290,8 +290,8 @@ int Engine::alloc(uint32_t cap, uint32_t tcap, std::string *err)
         return check(hipMalloc(p, bytes), what, err);
     };
     const size_t f4 = sizeof(float4) * (size_t)cap, u4 = sizeof(uint32_t) * (size_t)cap;
-    // radix blocks: each block owns at least 16 tiles of 256 keys
-    uint32_t nblk = (cap + 4095) / 4096;
+    // radix blocks: at least 4 tiles of 256 keys each, at most 1024 blocks
+    uint32_t nblk = (cap + 1023) / 1024;
     nblk = std::max<uint32_t>(1, std::min<uint32_t>(nblk, 1024));
     d_.radix_blocks = nblk;
     d_.capacity = cap;

@@ -79,9 +79,11 @@ inline LaunchCfg default_launch_cfg(bool grid)
 {
     LaunchCfg c;
     if (grid) {
-        c.density = GatherVariant{false, false, 256};
-        c.pressure = GatherVariant{false, true, 128};
-        c.viscosity = GatherVariant{false, false, 256};
+        // the autotuner's usual picks: neighbour masks for all three kernels
+        c.density = GatherVariant{false, false, 256, true};
+        c.pressure = GatherVariant{false, false, 256, true};
+        c.viscosity = GatherVariant{false, false, 128, true};
+        c.nbr = true;
     }
     return sanitize_launch_cfg(c, grid);
 }
