@@ -761,13 +761,14 @@ __device__ __forceinline__ void gather(const SceneDesc &S, const float4 *__restr
     const float4 pi = xps[k];                                                       \
     uint32_t *lst = lds_list + threadIdx.x;                                         \
     /* GRID: each wave's row-staging buffer follows the lists */                    \
-    float4 *stage = GRID ? reinterpret_cast<float4 *>(lds_list + (LIST ? kListMax * blockDim.x : 0)) + \
-                               (threadIdx.x >> 6) * kStageBuf                       \
-                         : nullptr;                                                 \
-    /* GRID + NBR consumers: each wave's neighbour-mask table follows */           \
-    uint32_t *mtab = reinterpret_cast<uint32_t *>(reinterpret_cast<float4 *>(       \
-                         lds_list + (LIST ? kListMax * blockDim.x : 0)) +           \
-                         (blockDim.x >> 6) * kStageBuf) + (threadIdx.x >> 6) * kMaskTabWords; \
+    /* GRID: each wave's row-staging buffer follows the lists; the density   */  \
+    /* pass (CONSUMER false) stages rows, the mask consumers do not (their  */  \
+    /* rare fallback walk reads global memory) and hold a mask table instead */ \
+    float4 *stage = (GRID && !CONSUMER)                                            \
+                        ? reinterpret_cast<float4 *>(lds_list + (LIST ? kListMax * blockDim.x : 0)) + \
+                              (threadIdx.x >> 6) * kStageBuf                        \
+                        : nullptr;                                                  \
+    uint32_t *mtab = lds_list + (threadIdx.x >> 6) * kMaskTabWords;                \
     (void)mtab;                                                                     \
     (void)g;                                                                        \
     (void)lst
@@ -827,6 +828,7 @@ __device__ __forceinline__ void gather_row(const SceneDesc &S, const float4 *__r
 template <bool MULTI, bool WORK, bool LIST, bool NBR, bool GRID>
 __global__ __launch_bounds__(256) void k_density(SPH_GATHER_PARAMS, float4 *__restrict__ dens)
 {
+    constexpr bool CONSUMER = false;
     SPH_GATHER_PROLOGUE();
     const float h = S.h, w2c = S.w2c, w3c = S.w3c;
     float rho = 0.0f, rhon = 0.0f;
@@ -912,6 +914,7 @@ __global__ __launch_bounds__(256) void k_pressure(SPH_GATHER_PARAMS, const float
                                                    const float4 *__restrict__ vs,
                                                    float4 *__restrict__ vs2)
 {
+    constexpr bool CONSUMER = NBR && GRID;  // reads the neighbour masks
     SPH_GATHER_PROLOGUE();
     const float4 di = dens[k];
     const float h = S.h, dw2c = S.dw2c, dw3c = S.dw3c, rho0 = S.rho0, kp = S.k, kn = S.k_near;
@@ -978,6 +981,7 @@ __global__ __launch_bounds__(256) void k_viscosity_integrate(SPH_GATHER_PARAMS,
                                                               float4 *__restrict__ vel,
                                                               float4 *__restrict__ colors)
 {
+    constexpr bool CONSUMER = NBR && GRID;  // reads the neighbour masks
     SPH_GATHER_PROLOGUE();
     const float4 vi = vs2[k];
     const float4 x0 = pos[g];  // issued early: used after the gather
@@ -1213,8 +1217,7 @@ inline uint32_t gather_grid(uint32_t cap, int block)
         const size_t lds_ = v_.list ? sizeof(uint32_t) * kListMax * (size_t)b_ : 0;           \
         const bool m_ = s.multi;                                                              \
         if (s.grid && c.nbr && v_.rows) {                                                     \
-            const size_t ldsr_ = (sizeof(float4) * kStageBuf + sizeof(uint32_t) * kMaskTabWords) * \
-                                 (size_t)(b_ / 64);                                           \
+            const size_t ldsr_ = sizeof(uint32_t) * kMaskTabWords * (size_t)(b_ / 64);       \
             if (m_) KER<true, false, false, true, true><<<g_, b_, ldsr_, st>>>(__VA_ARGS__);  \
             else KER<false, false, false, true, true><<<g_, b_, ldsr_, st>>>(__VA_ARGS__);    \
         } else if (s.grid) {                                                                  \
