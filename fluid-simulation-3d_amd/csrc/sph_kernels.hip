@@ -279,7 +279,7 @@ __global__ void k_reorder(const EngineDesc *__restrict__ ed, const uint32_t *__r
         // the reference's lookup record keeps the hash as a float (cc:480)
         xps[k] = make_float4(r.x, r.y, r.z, (float)__float_as_uint(r.w));
         vs[k] = vel[g];
-        rank[g] = k;
+        if (rank) rank[g] = k;  // only the Morton work order reads it
         const uint32_t prev = k ? skeys[k - 1] : 0xffffffffu;
         const uint32_t next = k + 1 < n ? skeys[k + 1] : 0xffffffffu;
         if (key != prev) start[key] = k;
@@ -1196,7 +1196,8 @@ hipError_t launch_radix_sort(const DevState &s, uint32_t *const keys[2], uint32_
 hipError_t launch_reorder(const DevState &s, const LaunchCfg &c, int buf, hipStream_t st)
 {
     k_reorder<<<grid_for(s.capacity, c.block_stream), c.block_stream, 0, st>>>(
-        s.ed, s.keys[buf], s.vals[buf], s.xp, s.vel, s.xps, s.vs, s.start, s.end, s.rank);
+        s.ed, s.keys[buf], s.vals[buf], s.xp, s.vel, s.xps, s.vs, s.start, s.end,
+        s.grid ? nullptr : s.rank);
     SPH_LAUNCH_CHECK();
 }
 
