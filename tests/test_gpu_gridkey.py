@@ -211,3 +211,27 @@ def test_grid_key_batch_equals_oracle():
                 assert_bitexact(got[s][0][:, :3], pos_o[:, :3], f"scene {s} pos list {l} masks {nb}")
                 assert_bitexact(got[s][1][:, :3], vel_o[:, :3], f"scene {s} vel list {l} masks {nb}")
                 assert_bitexact(got[s][2], dens_o, f"scene {s} dens list {l} masks {nb}")
+
+
+def test_grid_key_settings_change_mid_run():
+    """setInteractionRadius / setBound between updates (physicsWorld.h:56-77):
+    the grid periods are recomputed inside the handle's fixed bucket count, so
+    a smaller h makes cells wrap.  Results stay bit-exact."""
+    sc = W.dam_break(n=8192)
+    dt = _sub_dt()
+    op0 = orc_params(sc.params)
+    p1 = W.scene_params(h=0.15, box=(5.0, 4.0, 6.0))  # dam_break's settings otherwise
+    with S.Sim(sc.params, sc.n, S.TABLE_GRID) as sim:
+        g0 = sim.key()[1]
+        sim.upload(sc.pos, sc.vel)
+        sim.step(W.DT, W.SUBSTEPS)
+        sim.params = p1
+        g1 = sim.key()[1]
+        sim.step(W.DT, W.SUBSTEPS)
+        pos_g, vel_g, dens_g = sim.download()
+    assert g0 != g1
+    pos_o, vel_o, _ = O.update(op0, dt, sc.pos, sc.vel, T=sc.n, substeps=W.SUBSTEPS)
+    pos_o, vel_o, dens_o = O.update(orc_params(p1), dt, pos_o, vel_o, T=sc.n, substeps=W.SUBSTEPS)
+    assert_bitexact(pos_g[:, :3], pos_o[:, :3], "positions after a settings change")
+    assert_bitexact(vel_g[:, :3], vel_o[:, :3], "velocities after a settings change")
+    assert_bitexact(dens_g, dens_o, "densities after a settings change")
