@@ -231,6 +231,7 @@ int Engine::create(const std::vector<SceneSpec> &scenes, uint32_t capacity,
         fill_scene_desc(scenes[s].params, scenes[s].n, scenes[s].table, grid, &d);
         d.p_off = p_off;
         d.key_off = k_off;
+        d.index = s;
         d.dt = 0.0f;
         for (uint32_t i = 0; i < scenes[s].n; ++i) scene_of[p_off + i] = s;
         p_off += scenes[s].n;
@@ -325,6 +326,9 @@ int Engine::alloc(uint32_t cap, uint32_t tcap, std::string *err)
     if (!rc) rc = dm((void **)&d_.rank, u4, "rank");
     if (!rc) rc = dm((void **)&d_.nbr, u4 * 64, "neighbour rows");
     if (!rc) rc = dm((void **)&d_.ncnt, u4, "neighbour counts");
+    if (!rc) rc = dm((void **)&d_.sflags, sizeof(uint32_t) * scene_cap_, "scene flags");
+    // wrapped until a predict pass says otherwise
+    if (!rc) rc = check(hipMemset(d_.sflags, 0xff, sizeof(uint32_t) * scene_cap_), "scene flags", err);
     d_.ed = d_ed_;
     d_.sd = d_sd_;
     d_.scene_of = d_scene_of_;
@@ -340,7 +344,7 @@ Engine::~Engine()
                     d_.xps,      d_.vs,       d_.vs2,      d_.dens,    d_.keys[0], d_.keys[1],
                     d_.vals[0],  d_.vals[1],  d_.hist,     d_.start,   d_.end,
                     d_.wkeys[0], d_.wkeys[1], d_.wvals[0], d_.wvals[1], d_.rank,
-                    d_.nbr,      d_.ncnt};
+                    d_.nbr,      d_.ncnt,     d_.sflags};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (tmp_pos_) (void)hipFree(tmp_pos_);
@@ -451,6 +455,7 @@ int Engine::repack(const std::vector<RepackItem> &items, std::string *err)
         const RepackItem &it = items[q];
         fill_scene_desc(it.spec.params, it.spec.n, it.spec.table, grid_, &sd[q]);
         sd[q].p_off = new_off[q];
+        sd[q].index = (uint32_t)q;
         sd[q].key_off = k_off;
         sd[q].dt = dt_cached_ < 0.0f ? 0.0f : dt_cached_;
         k_off += it.spec.table;
@@ -499,6 +504,7 @@ int Engine::set_params(uint32_t s, const sph_params &p, std::string *err)
     fill_scene_desc(p, d.n, d.table, grid_, &d);
     d.p_off = p_off;
     d.key_off = k_off;
+    d.index = s;
     d.dt = dt;
     params_[s] = p;
     return push_descs(err);
@@ -580,6 +586,8 @@ hipError_t Engine::enqueue_update(hipEvent_t *ev, bool rebuild_work)
 #define LK(x)                              \
     if ((e = (x)) != hipSuccess) return e;
     if (ev) LK(hipEventRecord(ev[0], stream_));
+    // the table clear also resets the per-scene flags that predict raises
+    LK(launch_clear_table(d_, cfg_, stream_));
     LK(launch_predict(d_, cfg_, stream_));
     if (ev) LK(hipEventRecord(ev[1], stream_));
     if (rebuild_work) {
@@ -588,7 +596,6 @@ hipError_t Engine::enqueue_update(hipEvent_t *ev, bool rebuild_work)
         d_.work = d_.wvals[wb];
         work_valid_ = true;
     }
-    LK(launch_clear_table(d_, cfg_, stream_));
     LK(launch_radix_sort(d_, d_.keys, d_.vals, key_bits_, stream_, &sorted_buf_));
     LK(launch_reorder(d_, cfg_, sorted_buf_, stream_));
     if (ev) LK(hipEventRecord(ev[2], stream_));
@@ -826,6 +833,7 @@ int Engine::prime(std::string *err)
     // dt = 0 reproduces xp = x + 0 exactly.
     int rc = set_dt(0.0f, err);
     if (rc) return rc;
+    CK(launch_clear_table(d_, cfg_, stream_), "prime clear");
     CK(launch_predict(d_, cfg_, stream_), "prime predict");
     if (cfg_.needs_work()) {
         int wb = 0;
@@ -833,7 +841,6 @@ int Engine::prime(std::string *err)
         d_.work = d_.wvals[wb];
         work_valid_ = true;
     }
-    CK(launch_clear_table(d_, cfg_, stream_), "prime clear");
     CK(launch_radix_sort(d_, d_.keys, d_.vals, key_bits_, stream_, &sorted_buf_), "prime sort");
     CK(launch_reorder(d_, cfg_, sorted_buf_, stream_), "prime reorder");
     CK(launch_density(d_, cfg_, sorted_buf_, stream_), "prime density");

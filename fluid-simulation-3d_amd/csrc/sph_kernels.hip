@@ -86,7 +86,8 @@ template <bool MULTI>
 __global__ void k_predict(const EngineDesc *__restrict__ ed, const SceneDesc *__restrict__ sd,
                           const uint32_t *__restrict__ scene_of, const float4 *__restrict__ pos,
                           float4 *__restrict__ vel, float4 *__restrict__ xp,
-                          uint32_t *__restrict__ keys, uint32_t *__restrict__ wkeys)
+                          uint32_t *__restrict__ keys, uint32_t *__restrict__ wkeys,
+                          uint32_t *__restrict__ sflags)
 {
     const EngineDesc E = *ed;
     for (uint32_t g = global_id(); g < E.n_total; g += grid_stride()) {
@@ -110,14 +111,21 @@ __global__ void k_predict(const EngineDesc *__restrict__ ed, const SceneDesc *__
         // bucket: GetKeyFromHash (cc:513-516), or the cell-grid key
         keys[g] = S.key_off + (S.grid ? grid_key(cx, cy, cz, S.cell_lo, S.gdim, S.gmod)
                                       : fastmod(hash, S.mod_m, S.table));
+        // grid key: a cell outside the grid's box wraps onto another cell's bucket
+        if (S.grid && ((uint32_t)cx - (uint32_t)S.cell_lo[0] >= S.gdim[0] ||
+                       (uint32_t)cy - (uint32_t)S.cell_lo[1] >= S.gdim[1] ||
+                       (uint32_t)cz - (uint32_t)S.cell_lo[2] >= S.gdim[2]))
+            sflags[S.index] = 1u;
         wkeys[g] = work_key(E, S, s, cx, cy, cz);
     }
 }
 
-__global__ void k_fill(const EngineDesc *__restrict__ ed, uint32_t *__restrict__ a, uint32_t v)
+__global__ void k_fill(const EngineDesc *__restrict__ ed, uint32_t *__restrict__ a, uint32_t v,
+                       uint32_t *__restrict__ sflags)
 {
     const uint32_t n = ed->table_total;
     for (uint32_t i = global_id(); i < n; i += grid_stride()) a[i] = v;
+    if (global_id() < ed->n_scenes) sflags[global_id()] = 0u;  // raised again by predict
 }
 
 // ---------------------------------------------------------------------------
@@ -394,7 +402,7 @@ constexpr uint32_t kStageBuf = kStageCap + KBG; // + padding: a batch may read K
 // branches.  STAGED: the records are read from the wave's LDS copy
 // src[slot - base] of a range containing [b, e) (the padded buffer makes the
 // reads past e harmless; they never pass); else from xps in global memory.
-template <bool STAGED, bool SKIP_SELF, bool AUX, bool MASKS, typename Body>
+template <bool STAGED, bool SKIP_SELF, bool AUX, bool MASKS, bool HASH, typename Body>
 __device__ __forceinline__ void walk_run(const float4 *src, uint32_t base, const float4 *__restrict__ xps,
                                          const float4 *__restrict__ aux, uint32_t k,
                                          const float4 &pi, float thr, uint32_t b, uint32_t e,
@@ -423,7 +431,7 @@ __device__ __forceinline__ void walk_run(const float4 *src, uint32_t base, const
             d2[j] = ox[j] * ox[j] + oy[j] * oy[j] + oz[j] * oz[j];  // glm::dot order
             const float w = rr[j].w;
             pass[j] = ((int)(!STAGED || mj < e) & (int)(d2[j] < thr) &
-                       ((int)(w == g0) | (int)(w == g1) | (int)(w == g2)) &
+                       (int)(!HASH || (w == g0) | (w == g1) | (w == g2)) &
                        (int)(!SKIP_SELF || mj != k)) != 0;
         }
         float4 a[KBG];
@@ -461,7 +469,7 @@ __device__ __forceinline__ void walk_grid(const SceneDesc &S, const float4 *__re
                                           const float4 *__restrict__ aux,
                                           const uint32_t *__restrict__ start,
                                           const uint32_t *__restrict__ endt, uint32_t k,
-                                          const float4 &pi, float4 *stage, Body &&body,
+                                          const float4 &pi, float4 *stage, bool clean, Body &&body,
                                           const MaskOut &mo = MaskOut{nullptr, nullptr, 0})
 {
     const int cx = cell_coord(pi.x, S.h), cy = cell_coord(pi.y, S.h), cz = cell_coord(pi.z, S.h);
@@ -471,6 +479,12 @@ __device__ __forceinline__ void walk_grid(const SceneDesc &S, const float4 *__re
     const uint32_t ux = (uint32_t)cx - (uint32_t)S.cell_lo[0];
     const uint32_t uy = (uint32_t)cy - (uint32_t)S.cell_lo[1];
     const uint32_t uz = (uint32_t)cz - (uint32_t)S.cell_lo[2];
+    // No hash test needed: when no particle of the scene has its cell outside
+    // the grid's box (clean) and this particle's 27 cells are all inside it,
+    // the key is injective on the occupied cells, so each of those cells owns
+    // its bucket and every record of a row's run belongs to the row.
+    const bool inner = clean && ux - 1u < S.gdim[0] - 2u && uy - 1u < gy - 2u && uz - 1u < gz - 2u;
+    const bool nohash = __ballot(inner) == __ballot(1);
     uint32_t wx[3], wy[3], wz[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -528,12 +542,15 @@ __device__ __forceinline__ void walk_grid(const SceneDesc &S, const float4 *__re
             auto inline_body = [&](uint32_t m, float ox, float oy, float oz, float d2, const float4 &a) {
                 if (!defer) body(m, ox, oy, oz, d2, a);
             };
-            if (staged)
-                walk_run<true, SKIP_SELF, AUX, MASKS || DEFER>(stage, lo, xps, aux, k, pi, thr, rb, re, f0, f1, f2,
-                                                              mask, inline_body);
+            if (staged && nohash)
+                walk_run<true, SKIP_SELF, AUX, MASKS || DEFER, false>(stage, lo, xps, aux, k, pi, thr, rb, re,
+                                                                     f0, f1, f2, mask, inline_body);
+            else if (staged)
+                walk_run<true, SKIP_SELF, AUX, MASKS || DEFER, true>(stage, lo, xps, aux, k, pi, thr, rb, re,
+                                                                    f0, f1, f2, mask, inline_body);
             else
-                walk_run<false, SKIP_SELF, AUX, MASKS || DEFER>(xps, 0, xps, aux, k, pi, thr, rb, re, f0, f1, f2,
-                                                               mask, inline_body);
+                walk_run<false, SKIP_SELF, AUX, MASKS || DEFER, true>(xps, 0, xps, aux, k, pi, thr, rb, re,
+                                                                     f0, f1, f2, mask, inline_body);
             if (DEFER && defer) {
                 uint64_t mm = mask;
                 const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -573,7 +590,8 @@ __device__ __forceinline__ void walk_grid(const SceneDesc &S, const float4 *__re
                 const uint32_t e = nq ? (q == 0 ? e0 : (q == 1 ? e1 : e2)) : 0u;
                 const float fq = q == 0 ? f0 : (q == 1 ? f1 : f2);
                 uint64_t unused = 0;
-                walk_run<false, SKIP_SELF, AUX, false>(xps, 0, xps, aux, k, pi, thr, b, e, fq, fq, fq, unused, body);
+                walk_run<false, SKIP_SELF, AUX, false, true>(xps, 0, xps, aux, k, pi, thr, b, e, fq, fq, fq, unused,
+                                                             body);
             }
         }
     }
@@ -652,10 +670,11 @@ __device__ __forceinline__ void walk(const SceneDesc &S, const float4 *__restric
                                      const float4 *__restrict__ aux,
                                      const uint32_t *__restrict__ start,
                                      const uint32_t *__restrict__ endt, uint32_t k,
-                                     const float4 &pi, float4 *stage, Body &&body,
+                                     const float4 &pi, float4 *stage, bool clean, Body &&body,
                                      const MaskOut &mo = MaskOut{nullptr, nullptr, 0})
 {
-    if constexpr (GRID) walk_grid<SKIP_SELF, AUX, MASKS, DEFER>(S, xps, aux, start, endt, k, pi, stage, body, mo);
+    if constexpr (GRID)
+        walk_grid<SKIP_SELF, AUX, MASKS, DEFER>(S, xps, aux, start, endt, k, pi, stage, clean, body, mo);
     else walk_hash<SKIP_SELF, AUX>(S, xps, aux, start, endt, k, pi, body);
 }
 
@@ -701,16 +720,16 @@ __device__ __forceinline__ void gather(const SceneDesc &S, const float4 *__restr
                                        const float4 *__restrict__ aux,
                                        const uint32_t *__restrict__ start,
                                        const uint32_t *__restrict__ endt, uint32_t *lst,
-                                       float4 *stage, uint32_t k, const float4 &pi, Body &&body,
-                                       const MaskOut &mo = MaskOut{nullptr, nullptr, 0})
+                                       float4 *stage, bool clean, uint32_t k, const float4 &pi,
+                                       Body &&body, const MaskOut &mo = MaskOut{nullptr, nullptr, 0})
 {
     if (!LIST) {
-        walk<GRID, SKIP_SELF, AUX, MASKS>(S, xps, aux, start, endt, k, pi, stage, body, mo);
+        walk<GRID, SKIP_SELF, AUX, MASKS>(S, xps, aux, start, endt, k, pi, stage, clean, body, mo);
         return;
     }
     const uint32_t stride = blockDim.x;
     uint32_t c = 0;
-    walk<GRID, SKIP_SELF, false, MASKS>(S, xps, nullptr, start, endt, k, pi, stage,
+    walk<GRID, SKIP_SELF, false, MASKS>(S, xps, nullptr, start, endt, k, pi, stage, clean,
                            [&](uint32_t m, float, float, float, float, const float4 &) {
                                if (c < kListMax) lst[c * stride] = m;
                                ++c;
@@ -742,7 +761,7 @@ __device__ __forceinline__ void gather(const SceneDesc &S, const float4 *__restr
     }
     if (c > kListMax) {
         uint32_t skip = kListMax;
-        walk<GRID, SKIP_SELF, AUX>(S, xps, aux, start, endt, k, pi, stage,
+        walk<GRID, SKIP_SELF, AUX>(S, xps, aux, start, endt, k, pi, stage, clean,
                              [&](uint32_t m, float ox, float oy, float oz, float d2,
                                  const float4 &a) {
                                  if (skip) --skip;
@@ -769,7 +788,10 @@ __device__ __forceinline__ void gather(const SceneDesc &S, const float4 *__restr
                               (threadIdx.x >> 6) * kStageBuf                        \
                         : nullptr;                                                  \
     uint32_t *mtab = lds_list + (threadIdx.x >> 6) * kMaskTabWords;                \
+    /* grid key: no particle of this scene has its cell outside the grid's box */ \
+    const bool clean = GRID && sflags[S.index] == 0u;                              \
     (void)mtab;                                                                     \
+    (void)clean;                                                                    \
     (void)g;                                                                        \
     (void)lst
 
@@ -779,7 +801,7 @@ __device__ __forceinline__ void gather(const SceneDesc &S, const float4 *__restr
         const uint32_t *__restrict__ rank, const uint32_t *__restrict__ svals,                \
         const float4 *__restrict__ xps, const uint32_t *__restrict__ start,                   \
         const uint32_t *__restrict__ endt, uint32_t *__restrict__ nbr,                        \
-        uint32_t *__restrict__ ncnt
+        uint32_t *__restrict__ ncnt, const uint32_t *__restrict__ sflags
 
 // NBR (with WORK): the density pass also stores each particle's neighbours
 // with d < h (self excluded, walk order) as row i = work index of the
@@ -799,7 +821,7 @@ __device__ __forceinline__ void gather_row(const SceneDesc &S, const float4 *__r
                                            uint32_t k, const float4 &pi, Body &&body)
 {
     if (c > kNbrMax) {
-        walk<false, true, true>(S, xps, aux, start, endt, k, pi, nullptr, body);
+        walk<false, true, true>(S, xps, aux, start, endt, k, pi, nullptr, false, body);
         return;
     }
     const uint4 *row4 = reinterpret_cast<const uint4 *>(row);
@@ -849,10 +871,10 @@ __global__ __launch_bounds__(256) void k_density(SPH_GATHER_PARAMS, float4 *__re
     // MASKS: the walk also records the neighbour masks.  Grid key without
     // the LDS list: the body runs over each row's mask (DEFER).
     if (GRID && !LIST)
-        walk<GRID, false, false, MASKS, true>(S, xps, nullptr, start, endt, k, pi, stage, body,
+        walk<GRID, false, false, MASKS, true>(S, xps, nullptr, start, endt, k, pi, stage, clean, body,
                                               MaskOut{nbr, ncnt, ed->n_total});
     else
-        gather<GRID, LIST, false, false, MASKS>(S, xps, nullptr, start, endt, lst, stage, k, pi, body,
+        gather<GRID, LIST, false, false, MASKS>(S, xps, nullptr, start, endt, lst, stage, clean, k, pi, body,
                                                 MaskOut{nbr, ncnt, ed->n_total});
     if (ROWS) ncnt[i] = c;
     // pressures of cc:371-372 / 407-408, computed once per particle
@@ -883,7 +905,7 @@ __global__ __launch_bounds__(256) void k_density_rows(SPH_GATHER_PARAMS, float4 
         const float4 pi = xps[k];
         const float h = S.h, w2c = S.w2c, w3c = S.w3c;
         float rho = 0.0f, rhon = 0.0f;
-        walk<false, false, false>(S, xps, nullptr, start, endt, k, pi, nullptr,
+        walk<false, false, false>(S, xps, nullptr, start, endt, k, pi, nullptr, false,
                            [&](uint32_t m, float, float, float, float d2, const float4 &) {
             const float d = sqrtf(d2);
             const float v = h - d;
@@ -960,11 +982,11 @@ __global__ __launch_bounds__(256) void k_pressure(SPH_GATHER_PARAMS, const float
     };
     if (NBR && GRID) {
         if (ncnt[k]) gather_masks<KB, true, true>(xps, dens, nbr, ed->n_total, k, pi, mtab, body);
-        else gather<GRID, false, true, true>(S, xps, dens, start, endt, lst, stage, k, pi, body);
+        else gather<GRID, false, true, true>(S, xps, dens, start, endt, lst, stage, clean, k, pi, body);
     } else if (NBR) {
         gather_row(S, xps, dens, start, endt, nbr + (size_t)i * kNbrMax, ncnt[i], k, pi, body);
     } else {
-        gather<GRID, LIST, true, true>(S, xps, dens, start, endt, lst, stage, k, pi, body);
+        gather<GRID, LIST, true, true>(S, xps, dens, start, endt, lst, stage, clean, k, pi, body);
     }
     float4 v = vs[k];
     const float dt = S.dt, rho = di.x;
@@ -997,11 +1019,11 @@ __global__ __launch_bounds__(256) void k_viscosity_integrate(SPH_GATHER_PARAMS,
     };
     if (NBR && GRID) {
         if (ncnt[k]) gather_masks<8, true, true>(xps, vs2, nbr, ed->n_total, k, pi, mtab, body);
-        else gather<GRID, false, true, true>(S, xps, vs2, start, endt, lst, stage, k, pi, body);
+        else gather<GRID, false, true, true>(S, xps, vs2, start, endt, lst, stage, clean, k, pi, body);
     } else if (NBR) {
         gather_row(S, xps, vs2, start, endt, nbr + (size_t)i * kNbrMax, ncnt[i], k, pi, body);
     } else {
-        gather<GRID, LIST, true, true>(S, xps, vs2, start, endt, lst, stage, k, pi, body);
+        gather<GRID, LIST, true, true>(S, xps, vs2, start, endt, lst, stage, clean, k, pi, body);
     }
     const float dt = S.dt, mu = S.mu;
     float4 v = vi;
@@ -1160,17 +1182,17 @@ hipError_t launch_predict(const DevState &s, const LaunchCfg &c, hipStream_t st)
     const uint32_t g = grid_for(s.capacity, c.block_stream);
     if (s.multi)
         k_predict<true><<<g, c.block_stream, 0, st>>>(s.ed, s.sd, s.scene_of, s.pos, s.vel, s.xp,
-                                                       s.keys[0], s.wkeys[0]);
+                                                       s.keys[0], s.wkeys[0], s.sflags);
     else
         k_predict<false><<<g, c.block_stream, 0, st>>>(s.ed, s.sd, s.scene_of, s.pos, s.vel, s.xp,
-                                                        s.keys[0], s.wkeys[0]);
+                                                        s.keys[0], s.wkeys[0], s.sflags);
     SPH_LAUNCH_CHECK();
 }
 
 hipError_t launch_clear_table(const DevState &s, const LaunchCfg &c, hipStream_t st)
 {
     k_fill<<<grid_for(s.table_capacity, c.block_stream), c.block_stream, 0, st>>>(
-        s.ed, s.start, SPH_START_EMPTY);
+        s.ed, s.start, SPH_START_EMPTY, s.sflags);
     SPH_LAUNCH_CHECK();
 }
 
@@ -1242,7 +1264,8 @@ inline uint32_t gather_grid(uint32_t cap, int block)
     } while (0)
 
 #define SPH_GATHER_ARGS                                                                \
-    s.ed, s.sd, s.scene_of, s.work, s.rank, s.vals[buf], s.xps, s.start, s.end, s.nbr, s.ncnt
+    s.ed, s.sd, s.scene_of, s.work, s.rank, s.vals[buf], s.xps, s.start, s.end, s.nbr, s.ncnt,   \
+        s.sflags
 
 hipError_t launch_density(const DevState &s, const LaunchCfg &c, int buf, hipStream_t st)
 {

@@ -39,6 +39,7 @@ struct DevState {
     const uint32_t *work;  // = wvals[result] of the last rebuild
     uint32_t *rank;      // [capacity] sorted slot of each particle id
     float4 *colors;      // [capacity] speed colours, id order (null: not written)
+    uint32_t *sflags;    // [scenes] grid key: 1 when a particle's cell lies outside the grid's box
     uint32_t *nbr;       // [capacity * 64] neighbour rows by work index (nbr mode)
     uint32_t *ncnt;      // [capacity] neighbour count per row
     uint32_t capacity;       // particle slots allocated

@@ -53,7 +53,7 @@ struct SceneDesc {
     int32_t cell_lo[3];   // origin of the scene's cell grid (grid key, work-order key)
     uint32_t grid;        // bucket key: 1 = cell-grid key, 0 = hash % table (cc:513-516)
     uint32_t gdim[3];     // grid key: period of the cell grid per axis, product <= table
-    uint32_t pad;
+    uint32_t index;       // this scene's position in the layout (per-scene flags)
     uint64_t gmod[3];     // grid key: remainder constants of gdim
 };
 

@@ -141,7 +141,9 @@ def _row_walk_lists(xp, p, gdim, lo, T):
     walk_grid / walk_run): 9 rows (dx outer, dy inner); where the three
     z-cells of a row are consecutive buckets their slots form one run and a
     record passes if d2 < thr and its float hash equals ANY of the three row
-    cells' float hashes; at the wrap seam the buckets are walked one by one.
+    cells' float hashes -- no hash test at all when no particle's cell lies
+    outside the grid's box and this particle's 27 cells are inside it; at the
+    wrap seam the buckets are walked one by one.
     Returns per-particle lists of passing particle indices, in walk order,
     self included, with the prefilter d2 <= r2 only (the neighbour list)."""
     h = np.float32(p.h)
@@ -164,10 +166,15 @@ def _row_walk_lists(xp, p, gdim, lo, T):
     def wrap(u, g):
         return (u % M) % g
 
+    # no particle's cell outside the grid's box: the key is injective on them
+    uall = [((cells[:, a] - int(lo[a])) % M) for a in range(3)]
+    clean = all(bool(np.all(uall[a] < G[a])) for a in range(3))
     out = []
     for i in range(n):
         c = cells[i]
         u = [(int(c[a]) - int(lo[a])) % M for a in range(3)]
+        # the GPU skips the hash test for such a particle whose 27 cells are inside
+        inner = clean and all(1 <= u[a] <= G[a] - 2 for a in range(3))
         wx = [wrap(u[0] + d - 1, G[0]) for d in range(3)]
         wy = [wrap(u[1] + d - 1, G[1]) for d in range(3)]
         wz = [wrap(u[2] + d - 1, G[2]) for d in range(3)]
@@ -193,7 +200,7 @@ def _row_walk_lists(xp, p, gdim, lo, T):
                 for m in range(b, e):
                     o = rec[m] - pi
                     d2 = np.float32(np.float32(o[0] * o[0]) + np.float32(o[1] * o[1])) + np.float32(o[2] * o[2])
-                    if d2 <= r2 and rech[m] in fs:
+                    if d2 <= r2 and (inner or rech[m] in fs):
                         lst.append(int(srt[m]))
         out.append(lst)
     return out
