@@ -64,6 +64,29 @@ def kernel_name(stage, variants, grid, multi=False):
     return f"{base}<{m}, {b(v['work'])}, {b(v['list'])}, false, false>"
 
 
+VALU_JSON = os.path.join(ROOT, "profiles", "uniform_1m_valu.json")
+
+
+def valu_issue(workload, kernel, ms):
+    """VALU issue fraction of `kernel` at duration `ms`, from its committed
+    instruction count (rocprofv3 SQ_INSTS_VALU): instructions x 2 cycles (a
+    wave64 VALU instruction on a 32-lane SIMD) / (1024 SIMDs x 2.4 GHz x ms)."""
+    try:
+        with open(VALU_JSON) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    if t.get("workload") != workload or k is None:
+        return None
+    instr = float(k["valu_instr_per_launch"])
+    cyc = t["cycles_per_wave64_valu"]
+    return {"kernel": kernel, "instr_per_launch": instr,
+            "issue_frac": instr * cyc / (t["simds"] * t["clock_ghz"] * 1e9 * ms * 1e-3),
+            "model": f"{cyc} cycles per wave64 VALU instruction, {t['simds']} SIMDs, "
+                     f"{t['clock_ghz']} GHz"}
+
+
 def measured_traffic(workload, kernel):
     """HBM bytes per launch of `kernel` from the committed counter summary, or None."""
     try:
@@ -326,6 +349,7 @@ def main():
                      "traffic": measured_traffic(sc.name, dom_kernel),
                      "alg_bytes_per_launch": ALG_BYTES[dom] * sc.n,
                      "alg_bytes_per_particle": ALG_BYTES[dom]},
+        "valu": valu_issue(sc.name, dom_kernel, per[dom]),
         "nonfinite_components": nonfinite,
     }
     if not args.no_batch:
