@@ -137,6 +137,16 @@ def test_two_particle_pressure_closed_form(d):
     assert v[0, 1] == 0 and v[0, 2] == 0
     if P > 0:
         assert v[0, 0] < 0                                      # pushed away from particle 1
+    # the FP64 restatement (used for tolerances) meets the same closed form tightly
+    d64 = O.density64(p, xp, s)
+    assert d64[0, 0] == pytest.approx(rho, rel=1e-12) and d64[0, 1] == pytest.approx(rhon, rel=1e-12)
+    dv64, mag64 = O.pressure64(p, dt, xp, s, d64)
+    assert dv64[0, 0] == pytest.approx(f0 / rho * dt, rel=1e-9)
+    assert dv64[1, 0] == pytest.approx(-dv64[0, 0], rel=1e-12)
+    assert dv64[0, 1] == 0 and dv64[0, 2] == 0
+    # the scale of the sum: |pressure term| + |near-pressure term|
+    assert mag64[0, 0] == pytest.approx((abs(dw2 * P / rho) + abs(dw3 * Pn / rhon)) / rho * dt,
+                                        rel=1e-9)
 
 
 def test_coincident_particles_push_along_plus_y():
