@@ -694,8 +694,13 @@ __device__ __forceinline__ void walk(const SceneDesc &S, const float4 *__restric
 //   keeps walk order (sums keep the reference's order); a particle with more
 //   than kListMax neighbours finishes with a second walk that skips the
 //   listed ones.  Without LIST the body runs inside the walk.
-// Which work-item handles a particle, and whether the body runs in the walk
-// or from the list, never changes a result.
+// NBR (row mode): the density pass records each particle's neighbours and
+//   pressure and viscosity take them from that record instead of walking:
+//   64-entry neighbour rows under the reference key (gather_row), per-row
+//   64-bit neighbour masks under the grid key (gather_masks).
+// GRID: the cell-grid bucket key (walk_grid), sorted slots in grid order.
+// Which work-item handles a particle, and whether the body runs in the walk,
+// from the list or from the row-mode record, never changes a result.
 constexpr uint32_t kListMax = 56;
 
 // GRID: work-item i handles sorted slot i (cells in grid order, already

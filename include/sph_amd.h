@@ -173,17 +173,21 @@ int sph_stage_ms(sph_handle h, double ms[6]);
 int sph_profile_begin(sph_handle h, uint32_t max_updates);
 int sph_profile_end(sph_handle h, double mean_ms[6], uint32_t *updates);
 /* Autotuner (no reference counterpart): times every variant of the three
- * gather kernels (work order on/off, LDS neighbour list on/off, workgroup
- * size 64/128/256, and the neighbour-row mode) on the current state with `updates` Update(dt) calls per
- * candidate, restoring positions and velocities after each, and keeps the
- * fastest variant per kernel.  All variants give bit-identical results.
- * ms (may be NULL) gets the chosen variants' mean ms (density, pressure,
- * viscosity).  Densities and debug tables are recomputed by the next update. */
+ * gather kernels (work order on/off under the reference key, LDS neighbour
+ * list on/off, workgroup size 64/128/256, and the row mode) on the current
+ * state with `updates` Update(dt) calls per candidate, restoring positions and
+ * velocities after each, and keeps the fastest variant per kernel.  All
+ * variants give bit-identical results.  ms (may be NULL) gets the chosen
+ * variants' mean ms (density, pressure, viscosity).  Densities and debug
+ * tables are recomputed by the next update.  Without it a handle runs the
+ * defaults (under the grid key: neighbour masks for all three kernels). */
 int sph_autotune(sph_handle h, float dt, int updates, double ms[3]);
 /* The variant of each gather kernel, q = 0 density, 1 pressure, 2 viscosity:
- * v[4q+0] work order, v[4q+1] LDS list, v[4q+2] workgroup size (64, 128,
- * 192, 256), v[4q+3] read neighbour rows (pressure, viscosity); v[12] row
- * mode (the density pass stores each particle's neighbour row). */
+ * v[4q+0] work order (reference key only; ignored under the grid key),
+ * v[4q+1] LDS list, v[4q+2] workgroup size (64, 128, 192, 256), v[4q+3] read
+ * the row-mode records (pressure, viscosity); v[12] row mode: the density
+ * pass records each particle's neighbours -- 64-entry neighbour rows under
+ * the reference key, per-row 64-bit neighbour masks under the grid key. */
 int sph_get_variants(sph_handle h, int32_t v[13]);
 int sph_set_variants(sph_handle h, const int32_t v[13]);
 /* Number of non-finite position/velocity components in the current state. */
