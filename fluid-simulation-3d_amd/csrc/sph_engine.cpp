@@ -295,6 +295,8 @@ int Engine::alloc(uint32_t cap, uint32_t tcap, std::string *err)
     uint32_t nblk = (cap + 1023) / 1024;
     nblk = std::max<uint32_t>(1, std::min<uint32_t>(nblk, 1024));
     d_.radix_blocks = nblk;
+    const uint32_t cblk = (uint32_t)(((uint64_t)tcap + 2047) / 2048);  // counting-sort chunks
+    d_.count_blocks = cblk;
     d_.capacity = cap;
     d_.table_capacity = tcap;
     int rc = 0;
@@ -317,8 +319,12 @@ int Engine::alloc(uint32_t cap, uint32_t tcap, std::string *err)
         if (!rc) rc = dm((void **)&d_.vals[b], u4, "vals");
     }
     if (!rc) rc = dm((void **)&d_.hist, sizeof(uint32_t) * 256 * ((size_t)nblk + 1), "radix hist");
-    if (!rc) rc = dm((void **)&d_.start, sizeof(uint32_t) * (size_t)tcap, "start table");
-    if (!rc) rc = dm((void **)&d_.end, sizeof(uint32_t) * (size_t)tcap, "end table");
+    if (!rc) rc = dm((void **)&d_.bounds, sizeof(uint32_t) * ((size_t)tcap + 1), "bucket bounds");
+    const size_t count_bytes = sizeof(uint32_t) * ((size_t)tcap + scene_cap_);
+    if (!rc) rc = dm((void **)&d_.count, count_bytes, "bucket counts");
+    if (!rc) rc = check(hipMemset(d_.count, 0, count_bytes), "bucket counts", err);
+    if (!rc) rc = dm((void **)&d_.parts, sizeof(uint32_t) * ((size_t)cblk + 2), "count chunks");
+    if (!rc) rc = check(hipMemset(d_.parts, 0, sizeof(uint32_t) * ((size_t)cblk + 2)), "count chunks", err);
     for (int b = 0; b < 2 && !rc; ++b) {
         rc = dm((void **)&d_.wkeys[b], u4, "work keys");
         if (!rc) rc = dm((void **)&d_.wvals[b], u4, "work order");
@@ -342,7 +348,7 @@ Engine::~Engine()
     invalidate_graph();
     void *bufs[] = {d_ed_,       d_sd_,       d_scene_of_, d_.pos,     d_.vel,    d_.xp,
                     d_.xps,      d_.vs,       d_.vs2,      d_.dens,    d_.keys[0], d_.keys[1],
-                    d_.vals[0],  d_.vals[1],  d_.hist,     d_.start,   d_.end,
+                    d_.vals[0],  d_.vals[1],  d_.hist,     d_.bounds,  d_.count,  d_.parts,
                     d_.wkeys[0], d_.wkeys[1], d_.wvals[0], d_.wvals[1], d_.rank,
                     d_.nbr,      d_.ncnt,     d_.sflags};
     for (void *b : bufs)
@@ -586,8 +592,6 @@ hipError_t Engine::enqueue_update(hipEvent_t *ev, bool rebuild_work)
 #define LK(x)                              \
     if ((e = (x)) != hipSuccess) return e;
     if (ev) LK(hipEventRecord(ev[0], stream_));
-    // the table clear also resets the per-scene flags that predict raises
-    LK(launch_clear_table(d_, cfg_, stream_));
     LK(launch_predict(d_, cfg_, stream_));
     if (ev) LK(hipEventRecord(ev[1], stream_));
     if (rebuild_work) {
@@ -596,8 +600,7 @@ hipError_t Engine::enqueue_update(hipEvent_t *ev, bool rebuild_work)
         d_.work = d_.wvals[wb];
         work_valid_ = true;
     }
-    LK(launch_radix_sort(d_, d_.keys, d_.vals, key_bits_, stream_, &sorted_buf_));
-    LK(launch_reorder(d_, cfg_, sorted_buf_, stream_));
+    LK(launch_bucket_sort(d_, cfg_, stream_));
     if (ev) LK(hipEventRecord(ev[2], stream_));
     LK(launch_density(d_, cfg_, sorted_buf_, stream_));
     if (ev) LK(hipEventRecord(ev[3], stream_));
@@ -833,16 +836,14 @@ int Engine::prime(std::string *err)
     // dt = 0 reproduces xp = x + 0 exactly.
     int rc = set_dt(0.0f, err);
     if (rc) return rc;
-    CK(launch_clear_table(d_, cfg_, stream_), "prime clear");
     CK(launch_predict(d_, cfg_, stream_), "prime predict");
-    if (cfg_.needs_work()) {
+    if (!grid_ && cfg_.needs_work()) {
         int wb = 0;
         CK(launch_radix_sort(d_, d_.wkeys, d_.wvals, 32, stream_, &wb), "prime work order");
         d_.work = d_.wvals[wb];
         work_valid_ = true;
     }
-    CK(launch_radix_sort(d_, d_.keys, d_.vals, key_bits_, stream_, &sorted_buf_), "prime sort");
-    CK(launch_reorder(d_, cfg_, sorted_buf_, stream_), "prime reorder");
+    CK(launch_bucket_sort(d_, cfg_, stream_), "prime sort");
     CK(launch_density(d_, cfg_, sorted_buf_, stream_), "prime density");
     CK(hipStreamSynchronize(stream_), "prime sync");
     have_tables_ = true;
@@ -896,32 +897,31 @@ int Engine::debug_spatial(uint32_t s, uint32_t *hash_id, uint32_t *key_id, uint3
     const SceneDesc &d = sd_h_[s];
     CK(hipStreamSynchronize(stream_), "sync");
     std::vector<float4> xp(d.n);
-    std::vector<uint32_t> sk(d.n), si(d.n);
+    std::vector<uint32_t> kid(d.n), si(d.n);
     CK(hipMemcpy(xp.data(), d_.xp + d.p_off, sizeof(float4) * d.n, hipMemcpyDeviceToHost), "xp");
-    CK(hipMemcpy(sk.data(), d_.keys[sorted_buf_] + d.p_off, sizeof(uint32_t) * d.n,
-                 hipMemcpyDeviceToHost), "sorted keys");
-    CK(hipMemcpy(si.data(), d_.vals[sorted_buf_] + d.p_off, sizeof(uint32_t) * d.n,
+    CK(hipMemcpy(kid.data(), d_.keys[0] + d.p_off, sizeof(uint32_t) * d.n,
+                 hipMemcpyDeviceToHost), "keys");
+    CK(hipMemcpy(si.data(), d_.vals[kSortedBuf] + d.p_off, sizeof(uint32_t) * d.n,
                  hipMemcpyDeviceToHost), "sorted idx");
     for (uint32_t k = 0; k < d.n; ++k) {
-        sk[k] -= d.key_off;
+        kid[k] -= d.key_off;
         si[k] -= d.p_off;
     }
     if (hash_id)
         for (uint32_t i = 0; i < d.n; ++i) memcpy(&hash_id[i], &xp[i].w, 4);
-    if (key_id)
-        for (uint32_t k = 0; k < d.n; ++k) key_id[si[k] < d.n ? si[k] : 0] = sk[k];
-    if (sorted_key) memcpy(sorted_key, sk.data(), sizeof(uint32_t) * d.n);
+    if (key_id) memcpy(key_id, kid.data(), sizeof(uint32_t) * d.n);
+    if (sorted_key)
+        for (uint32_t k = 0; k < d.n; ++k) sorted_key[k] = si[k] < d.n ? kid[si[k]] : 0xffffffffu;
     if (sorted_idx) memcpy(sorted_idx, si.data(), sizeof(uint32_t) * d.n);
     if (start || end) {
-        std::vector<uint32_t> st(d.table), en(d.table);
-        CK(hipMemcpy(st.data(), d_.start + d.key_off, sizeof(uint32_t) * d.table,
-                     hipMemcpyDeviceToHost), "start table");
-        CK(hipMemcpy(en.data(), d_.end + d.key_off, sizeof(uint32_t) * d.table,
-                     hipMemcpyDeviceToHost), "end table");
+        // the reference's table: INT_MAX start for an empty bucket (cc:481)
+        std::vector<uint32_t> bd((size_t)d.table + 1);
+        CK(hipMemcpy(bd.data(), d_.bounds + d.key_off, sizeof(uint32_t) * ((size_t)d.table + 1),
+                     hipMemcpyDeviceToHost), "bucket bounds");
         for (uint32_t b = 0; b < d.table; ++b) {
-            const bool empty = st[b] == SPH_START_EMPTY;
-            if (start) start[b] = empty ? SPH_START_EMPTY : st[b] - d.p_off;
-            if (end) end[b] = empty ? 0u : en[b] - d.p_off;
+            const bool empty = bd[b] == bd[b + 1];
+            if (start) start[b] = empty ? SPH_START_EMPTY : bd[b] - d.p_off;
+            if (end) end[b] = empty ? 0u : bd[b + 1] - d.p_off;
         }
     }
     return SPH_OK;

@@ -146,7 +146,7 @@ class Engine {
     uint32_t *d_scene_of_ = nullptr;
     uint32_t key_bits_ = 1;
     uint32_t scene_cap_ = 1;
-    int sorted_buf_ = 0;
+    int sorted_buf_ = kSortedBuf;
     bool have_tables_ = false;
     bool grid_ = false;  // cell-grid bucket key (else the reference's hash % table)
     bool use_graph_ = true;

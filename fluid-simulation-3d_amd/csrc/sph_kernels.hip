@@ -4,11 +4,12 @@
 // (FluidSimulation::Update, cc:39-111) and engine/physics/kernels.h.
 //
 // Pipeline of one Update(dt) over all scenes of an engine:
-//   k_predict          stage A + B1  (cc:42-48, 473-482), id order
-//   k_fill             bucket table <- INT_MAX (cc:481)
-//   k_radix_hist/scan/scatter  stage B2 (cc:484): stable LSD radix sort of
-//                      (key, slot); stability gives (key, particle index) order
-//   k_reorder          stage B3 (cc:486-496) + gather of xp, v into sorted order
+//   k_predict          stage A + B1  (cc:42-48, 473-482), id order; counts
+//                      the records of each bucket
+//   k_count_reduce/scan/scatter/reorder  stages B2 + B3 (cc:481-496): a
+//                      counting sort over the buckets, ordered within a bucket
+//                      by particle index (the stable (key, index) order);
+//                      bucket bounds; gather of xp, v into sorted order
 //   k_density          stage C (cc:304-311, 325-365)
 //   k_pressure         stage D (cc:62-69, 367-422)
 //   k_viscosity_integrate  stage E (cc:71-78, 424-464, Jacobi) + F (cc:80-108)
@@ -82,12 +83,16 @@ __device__ __forceinline__ uint32_t work_key(const EngineDesc &E, const SceneDes
 // ---------------------------------------------------------------------------
 // Stage A + B1
 // ---------------------------------------------------------------------------
+// count: [table_total] records per bucket (zero on entry; the atomic's old
+// value is the record's ticket, an arbitrary order within the bucket), then
+// [n_scenes] flags raised when a particle's cell lies outside the grid's box.
+// wkeys may be null (no Morton work order).
 template <bool MULTI>
 __global__ void k_predict(const EngineDesc *__restrict__ ed, const SceneDesc *__restrict__ sd,
                           const uint32_t *__restrict__ scene_of, const float4 *__restrict__ pos,
                           float4 *__restrict__ vel, float4 *__restrict__ xp,
-                          uint32_t *__restrict__ keys, uint32_t *__restrict__ wkeys,
-                          uint32_t *__restrict__ sflags)
+                          uint32_t *__restrict__ keys, uint32_t *__restrict__ ticks,
+                          uint32_t *__restrict__ count, uint32_t *__restrict__ wkeys)
 {
     const EngineDesc E = *ed;
     for (uint32_t g = global_id(); g < E.n_total; g += grid_stride()) {
@@ -109,29 +114,24 @@ __global__ void k_predict(const EngineDesc *__restrict__ ed, const SceneDesc *__
         vel[g] = v;
         xp[g] = make_float4(px, py, pz, __uint_as_float(hash));
         // bucket: GetKeyFromHash (cc:513-516), or the cell-grid key
-        keys[g] = S.key_off + (S.grid ? grid_key(cx, cy, cz, S.cell_lo, S.gdim, S.gmod)
-                                      : fastmod(hash, S.mod_m, S.table));
+        const uint32_t key = S.key_off + (S.grid ? grid_key(cx, cy, cz, S.cell_lo, S.gdim, S.gmod)
+                                                 : fastmod(hash, S.mod_m, S.table));
+        keys[g] = key;
+        ticks[g] = atomicAdd(&count[key], 1u);
         // grid key: a cell outside the grid's box wraps onto another cell's bucket
         if (S.grid && ((uint32_t)cx - (uint32_t)S.cell_lo[0] >= S.gdim[0] ||
                        (uint32_t)cy - (uint32_t)S.cell_lo[1] >= S.gdim[1] ||
                        (uint32_t)cz - (uint32_t)S.cell_lo[2] >= S.gdim[2]))
-            sflags[S.index] = 1u;
-        wkeys[g] = work_key(E, S, s, cx, cy, cz);
+            count[E.table_total + S.index] = 1u;
+        if (wkeys) wkeys[g] = work_key(E, S, s, cx, cy, cz);
     }
 }
 
-__global__ void k_fill(const EngineDesc *__restrict__ ed, uint32_t *__restrict__ a, uint32_t v,
-                       uint32_t *__restrict__ sflags)
-{
-    const uint32_t n = ed->table_total;
-    for (uint32_t i = global_id(); i < n; i += grid_stride()) a[i] = v;
-    if (global_id() < ed->n_scenes) sflags[global_id()] = 0u;  // raised again by predict
-}
-
 // ---------------------------------------------------------------------------
-// Stage B2: stable LSD radix sort.  A fixed grid of `nblk` blocks; block b owns
-// the contiguous chunk [b*chunk, min((b+1)*chunk, n)) and walks it tile by
-// tile in order, so equal digits keep their input order (stability).
+// Stable LSD radix sort (the Morton work order).  A fixed grid of `nblk`
+// blocks; block b owns the contiguous chunk [b*chunk, min((b+1)*chunk, n)) and
+// walks it tile by tile in order, so equal digits keep their input order
+// (stability).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void chunk_of(uint32_t n, uint32_t &beg, uint32_t &end)
 {
@@ -271,28 +271,159 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_scatter(
 }
 
 // ---------------------------------------------------------------------------
-// Stage B3 (cc:486-496) + gather into sorted order
+// Stages B2 + B3 (cc:481-496): counting sort over the buckets.  The reference
+// sorts (key, index) records with std::sort (cc:484) and takes each bucket's
+// first slot (cc:486-496); here the sorted order is the stable one, ascending
+// particle index within a bucket, and the table is the exclusive prefix sum of
+// the bucket counts: bucket b holds slots [bounds[b], bounds[b+1]), empty when
+// they are equal (the reference's INT_MAX start, cc:481).
+//   k_predict        count[key]++ and a ticket per particle (above)
+//   k_count_reduce   per-chunk sums; the last block to finish scans them and
+//                    publishes the scene flags
+//   k_count_scan     bounds from the chunk bases and the counts
+//   k_count_scatter  tmp[bounds[key] + ticket] = slot: each bucket's slots, in
+//                    ticket order
+//   k_count_reorder  a record's place in its bucket is the number of the
+//                    bucket's slots with a smaller index; gather into sorted
+//                    order; clear the counts for the next update
 // ---------------------------------------------------------------------------
-__global__ void k_reorder(const EngineDesc *__restrict__ ed, const uint32_t *__restrict__ skeys,
-                          const uint32_t *__restrict__ svals, const float4 *__restrict__ xp,
-                          const float4 *__restrict__ vel, float4 *__restrict__ xps,
-                          float4 *__restrict__ vs, uint32_t *__restrict__ start,
-                          uint32_t *__restrict__ endt, uint32_t *__restrict__ rank)
+constexpr uint32_t kCountPer = 8;                          // buckets per work-item
+constexpr uint32_t kCountChunk = kSortBlock * kCountPer;   // buckets per block
+
+__device__ __forceinline__ void load_counts(const uint32_t *__restrict__ count, uint32_t b0,
+                                            uint32_t T, uint32_t c[kCountPer])
+{
+    if (b0 + kCountPer <= T) {
+        const uint4 *q = reinterpret_cast<const uint4 *>(count + b0);
+        const uint4 u = q[0], v = q[1];
+        c[0] = u.x; c[1] = u.y; c[2] = u.z; c[3] = u.w;
+        c[4] = v.x; c[5] = v.y; c[6] = v.z; c[7] = v.w;
+    } else {
+#pragma unroll
+        for (uint32_t j = 0; j < kCountPer; ++j) c[j] = b0 + j < T ? count[b0 + j] : 0u;
+    }
+}
+
+// parts: [gridDim.x + 1]; done: a counter that is zero on entry and on exit.
+__global__ __launch_bounds__(kSortBlock) void k_count_reduce(const EngineDesc *__restrict__ ed,
+                                                              const uint32_t *__restrict__ count,
+                                                              uint32_t *__restrict__ parts,
+                                                              uint32_t *__restrict__ done,
+                                                              uint32_t *__restrict__ sflags)
+{
+    __shared__ uint32_t wsum[kSortWaves];
+    __shared__ uint32_t last;
+    const uint32_t T = ed->table_total, tid = threadIdx.x;
+    uint32_t c[kCountPer];
+    load_counts(count, blockIdx.x * kCountChunk + tid * kCountPer, T, c);
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kCountPer; ++j) sum += c[j];
+    uint32_t total;
+    (void)block_scan_excl(sum, wsum, &total);
+    if (tid == 0) {
+        parts[blockIdx.x] = total;
+        __threadfence();  // the sum is visible before the counter moves
+        last = atomicAdd(done, 1u) == gridDim.x - 1u;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();  // every other block's sum is visible here
+    // the last block: exclusive scan of the chunk sums, in place; each
+    // work-item scans a contiguous run of them
+    const uint32_t nb = gridDim.x;
+    const uint32_t per = (nb + kSortBlock - 1) / kSortBlock;
+    const uint32_t lo = min(nb, tid * per), hi = min(nb, lo + per);
+    uint32_t run = 0;
+    for (uint32_t i = lo; i < hi; ++i) run += parts[i];
+    uint32_t all;
+    uint32_t pre = block_scan_excl(run, wsum, &all);
+    for (uint32_t i = lo; i < hi; ++i) {
+        const uint32_t v = parts[i];
+        parts[i] = pre;
+        pre += v;
+    }
+    if (tid == 0) {
+        parts[nb] = all;
+        *done = 0u;
+    }
+    for (uint32_t q = tid; q < ed->n_scenes; q += kSortBlock) sflags[q] = count[T + q];
+}
+
+__global__ __launch_bounds__(kSortBlock) void k_count_scan(const EngineDesc *__restrict__ ed,
+                                                            const uint32_t *__restrict__ count,
+                                                            const uint32_t *__restrict__ parts,
+                                                            uint32_t *__restrict__ bounds)
+{
+    __shared__ uint32_t wsum[kSortWaves];
+    const uint32_t T = ed->table_total, tid = threadIdx.x;
+    const uint32_t b0 = blockIdx.x * kCountChunk + tid * kCountPer;
+    uint32_t c[kCountPer];
+    load_counts(count, b0, T, c);
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kCountPer; ++j) sum += c[j];
+    uint32_t total;
+    uint32_t p = parts[blockIdx.x] + block_scan_excl(sum, wsum, &total);
+    if (b0 + kCountPer <= T) {
+        uint4 *q = reinterpret_cast<uint4 *>(bounds + b0);
+        uint32_t o[kCountPer];
+#pragma unroll
+        for (uint32_t j = 0; j < kCountPer; ++j) {
+            o[j] = p;
+            p += c[j];
+        }
+        q[0] = make_uint4(o[0], o[1], o[2], o[3]);
+        q[1] = make_uint4(o[4], o[5], o[6], o[7]);
+    } else {
+#pragma unroll
+        for (uint32_t j = 0; j < kCountPer; ++j) {
+            if (b0 + j < T) bounds[b0 + j] = p;
+            p += c[j];
+        }
+    }
+    if (blockIdx.x == 0 && tid == 0) bounds[T] = parts[gridDim.x];
+}
+
+__global__ void k_count_scatter(const EngineDesc *__restrict__ ed,
+                                const uint32_t *__restrict__ keys,
+                                const uint32_t *__restrict__ ticks,
+                                const uint32_t *__restrict__ bounds, uint32_t *__restrict__ tmp)
+{
+    const uint32_t n = ed->n_total;
+    for (uint32_t g = global_id(); g < n; g += grid_stride()) {
+        const uint32_t slot = bounds[keys[g]] + ticks[g];  // < n: the counts started at zero
+        if (slot < n) tmp[slot] = g;
+    }
+}
+
+__global__ void k_count_reorder(const EngineDesc *__restrict__ ed,
+                                const uint32_t *__restrict__ keys,
+                                const uint32_t *__restrict__ bounds,
+                                const uint32_t *__restrict__ tmp, uint32_t *__restrict__ svals,
+                                const float4 *__restrict__ xp, const float4 *__restrict__ vel,
+                                float4 *__restrict__ xps, float4 *__restrict__ vs,
+                                uint32_t *__restrict__ rank, uint32_t *__restrict__ count)
 {
     const uint32_t n = ed->n_total;
     for (uint32_t k = global_id(); k < n; k += grid_stride()) {
-        const uint32_t key = skeys[k];
-        const uint32_t g = svals[k];
+        const uint32_t g = tmp[k];
+        if (g >= n) continue;  // never taken while the counts start at zero
+        const uint32_t key = keys[g];
+        const uint32_t lo = bounds[key], hi = bounds[key + 1];
+        uint32_t before = 0;
+        for (uint32_t j = lo; j < hi; ++j) before += tmp[j] < g ? 1u : 0u;
+        const uint32_t slot = lo + before;
+        if (slot >= hi) continue;  // as above
         const float4 r = xp[g];
+        svals[slot] = g;
         // the reference's lookup record keeps the hash as a float (cc:480)
-        xps[k] = make_float4(r.x, r.y, r.z, (float)__float_as_uint(r.w));
-        vs[k] = vel[g];
-        if (rank) rank[g] = k;  // only the Morton work order reads it
-        const uint32_t prev = k ? skeys[k - 1] : 0xffffffffu;
-        const uint32_t next = k + 1 < n ? skeys[k + 1] : 0xffffffffu;
-        if (key != prev) start[key] = k;
-        if (key != next) endt[key] = k + 1;
+        xps[slot] = make_float4(r.x, r.y, r.z, (float)__float_as_uint(r.w));
+        vs[slot] = vel[g];
+        if (rank) rank[g] = slot;  // only the Morton work order reads it
     }
+    const uint32_t nc = ed->table_total + ed->n_scenes;
+    for (uint32_t i = global_id(); i < nc; i += grid_stride()) count[i] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -329,7 +460,7 @@ __device__ __forceinline__ void walk_hash(const SceneDesc &S, const float4 *__re
         const uint32_t hash = hash_cell(cx + dx, cy + dy, cz + dz);
         const uint32_t key = key_off + fastmod(hash, mod_m, table);
         nb = start[key];
-        ne = endt[key];  // meaningful only when nb != SPH_START_EMPTY
+        ne = endt[key];
         nhf = (float)hash;
     };
     fetch(0);
@@ -338,7 +469,6 @@ __device__ __forceinline__ void walk_hash(const SceneDesc &S, const float4 *__re
         const uint32_t b = nb, e = ne;
         const float hf = nhf;
         if (o + 1 < 27) fetch(o + 1);
-        if (b == SPH_START_EMPTY) continue;
 #pragma unroll 1
         for (uint32_t m = b; m < e; m += KB) {
             float4 r[KB];
@@ -494,13 +624,20 @@ __device__ __forceinline__ void walk_grid(const SceneDesc &S, const float4 *__re
     }
     const bool run = wz[1] == wz[0] + 1u && wz[2] == wz[1] + 1u;
     const uint32_t h0 = hash_cell(cx, cy, cz);
-    uint32_t ns[3], ne[3];
+    // bucket bounds of the next row: the run [start of its first bucket, end of
+    // its last) when its buckets are consecutive, else each bucket's
+    uint32_t ns[3] = {0u, 0u, 0u}, ne[3] = {0u, 0u, 0u};
     auto fetch = [&](int r) {
         const uint32_t base = key_off + (wx[r / 3] * gy + wy[r % 3]) * gz;
+        if (run) {
+            ns[0] = start[base + wz[0]];
+            ne[2] = endt[base + wz[2]];
+        } else {
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            ns[q] = start[base + wz[q]];
-            ne[q] = endt[base + wz[q]];  // meaningful only when ns[q] != SPH_START_EMPTY
+            for (int q = 0; q < 3; ++q) {
+                ns[q] = start[base + wz[q]];
+                ne[q] = endt[base + wz[q]];
+            }
         }
     };
     fetch(0);
@@ -511,11 +648,9 @@ __device__ __forceinline__ void walk_grid(const SceneDesc &S, const float4 *__re
         // float hashes of the row's cells (hash_cell is linear mod 2^32)
         const uint32_t hr = h0 + (uint32_t)(r / 3 - 1) * 15823u + (uint32_t)(r % 3 - 1) * 9737333u;
         const float f0 = (float)(hr - 440817757u), f1 = (float)hr, f2 = (float)(hr + 440817757u);
-        const bool n0 = s0 != SPH_START_EMPTY, n1 = s1 != SPH_START_EMPTY, n2 = s2 != SPH_START_EMPTY;
         if (run) {
             // the row's merged run
-            const uint32_t rb = n0 ? s0 : (n1 ? s1 : (n2 ? s2 : 0u));
-            const uint32_t re = n2 ? e2 : (n1 ? e1 : (n0 ? e0 : 0u));
+            const uint32_t rb = s0, re = e2;
             bool staged = false;
             uint32_t lo = 0;
             if (stage) {
@@ -585,9 +720,8 @@ __device__ __forceinline__ void walk_grid(const SceneDesc &S, const float4 *__re
             // wrap seam: the three buckets one by one
 #pragma unroll 1
             for (int q = 0; q < 3; ++q) {
-                const bool nq = q == 0 ? n0 : (q == 1 ? n1 : n2);
-                const uint32_t b = nq ? (q == 0 ? s0 : (q == 1 ? s1 : s2)) : 0u;
-                const uint32_t e = nq ? (q == 0 ? e0 : (q == 1 ? e1 : e2)) : 0u;
+                const uint32_t b = q == 0 ? s0 : (q == 1 ? s1 : s2);
+                const uint32_t e = q == 0 ? e0 : (q == 1 ? e1 : e2);
                 const float fq = q == 0 ? f0 : (q == 1 ? f1 : f2);
                 uint64_t unused = 0;
                 walk_run<false, SKIP_SELF, AUX, false, true>(xps, 0, xps, aux, k, pi, thr, b, e, fq, fq, fq, unused,
@@ -1084,9 +1218,7 @@ __device__ __forceinline__ void walk_prefilter(const SceneDesc &S, const float4 
                 const uint32_t key =
                     S.key_off + (S.grid ? grid_key(cx + dx, cy + dy, cz + dz, S.cell_lo, S.gdim, S.gmod)
                                         : fastmod(hash, S.mod_m, S.table));
-                const uint32_t b = start[key];
-                if (b == SPH_START_EMPTY) continue;
-                const uint32_t e = endt[key];
+                const uint32_t b = start[key], e = endt[key];
                 const float hf = (float)hash;
                 for (uint32_t m = b; m < e; ++m) {
                     const float4 r = xps[m];
@@ -1185,19 +1317,13 @@ inline uint32_t grid_for(uint32_t n, int block)
 hipError_t launch_predict(const DevState &s, const LaunchCfg &c, hipStream_t st)
 {
     const uint32_t g = grid_for(s.capacity, c.block_stream);
+    uint32_t *const wk = s.grid ? nullptr : s.wkeys[0];  // no work order under the grid key
     if (s.multi)
         k_predict<true><<<g, c.block_stream, 0, st>>>(s.ed, s.sd, s.scene_of, s.pos, s.vel, s.xp,
-                                                       s.keys[0], s.wkeys[0], s.sflags);
+                                                       s.keys[0], s.keys[1], s.count, wk);
     else
         k_predict<false><<<g, c.block_stream, 0, st>>>(s.ed, s.sd, s.scene_of, s.pos, s.vel, s.xp,
-                                                        s.keys[0], s.wkeys[0], s.sflags);
-    SPH_LAUNCH_CHECK();
-}
-
-hipError_t launch_clear_table(const DevState &s, const LaunchCfg &c, hipStream_t st)
-{
-    k_fill<<<grid_for(s.table_capacity, c.block_stream), c.block_stream, 0, st>>>(
-        s.ed, s.start, SPH_START_EMPTY, s.sflags);
+                                                        s.keys[0], s.keys[1], s.count, wk);
     SPH_LAUNCH_CHECK();
 }
 
@@ -1220,11 +1346,16 @@ hipError_t launch_radix_sort(const DevState &s, uint32_t *const keys[2], uint32_
     SPH_LAUNCH_CHECK();
 }
 
-hipError_t launch_reorder(const DevState &s, const LaunchCfg &c, int buf, hipStream_t st)
+hipError_t launch_bucket_sort(const DevState &s, const LaunchCfg &c, hipStream_t st)
 {
-    k_reorder<<<grid_for(s.capacity, c.block_stream), c.block_stream, 0, st>>>(
-        s.ed, s.keys[buf], s.vals[buf], s.xp, s.vel, s.xps, s.vs, s.start, s.end,
-        s.grid ? nullptr : s.rank);
+    const uint32_t nb = s.count_blocks, g = grid_for(s.capacity, c.block_stream);
+    k_count_reduce<<<nb, kSortBlock, 0, st>>>(s.ed, s.count, s.parts, s.parts + nb + 1, s.sflags);
+    k_count_scan<<<nb, kSortBlock, 0, st>>>(s.ed, s.count, s.parts, s.bounds);
+    k_count_scatter<<<g, c.block_stream, 0, st>>>(s.ed, s.keys[0], s.keys[1], s.bounds,
+                                                   s.vals[0]);
+    k_count_reorder<<<g, c.block_stream, 0, st>>>(s.ed, s.keys[0], s.bounds, s.vals[0],
+                                                   s.vals[kSortedBuf], s.xp, s.vel, s.xps, s.vs,
+                                                   s.grid ? nullptr : s.rank, s.count);
     SPH_LAUNCH_CHECK();
 }
 
@@ -1269,7 +1400,7 @@ inline uint32_t gather_grid(uint32_t cap, int block)
     } while (0)
 
 #define SPH_GATHER_ARGS                                                                \
-    s.ed, s.sd, s.scene_of, s.work, s.rank, s.vals[buf], s.xps, s.start, s.end, s.nbr, s.ncnt,   \
+    s.ed, s.sd, s.scene_of, s.work, s.rank, s.vals[buf], s.xps, s.bounds, s.bounds + 1, s.nbr, s.ncnt,   \
         s.sflags
 
 hipError_t launch_density(const DevState &s, const LaunchCfg &c, int buf, hipStream_t st)
@@ -1328,10 +1459,10 @@ hipError_t launch_neighbour_count(const DevState &s, int buf, uint32_t *counts, 
 {
     const uint32_t g = grid_for(s.capacity, 256);
     if (s.multi)
-        k_neighbour_count<true><<<g, 256, 0, st>>>(s.ed, s.sd, s.scene_of, s.xps, s.start, s.end,
+        k_neighbour_count<true><<<g, 256, 0, st>>>(s.ed, s.sd, s.scene_of, s.xps, s.bounds, s.bounds + 1,
                                                    s.vals[buf], counts);
     else
-        k_neighbour_count<false><<<g, 256, 0, st>>>(s.ed, s.sd, s.scene_of, s.xps, s.start, s.end,
+        k_neighbour_count<false><<<g, 256, 0, st>>>(s.ed, s.sd, s.scene_of, s.xps, s.bounds, s.bounds + 1,
                                                     s.vals[buf], counts);
     SPH_LAUNCH_CHECK();
 }
@@ -1341,10 +1472,10 @@ hipError_t launch_neighbour_fill(const DevState &s, int buf, const uint64_t *off
 {
     const uint32_t g = grid_for(s.capacity, 256);
     if (s.multi)
-        k_neighbour_fill<true><<<g, 256, 0, st>>>(s.ed, s.sd, s.scene_of, s.xps, s.start, s.end,
+        k_neighbour_fill<true><<<g, 256, 0, st>>>(s.ed, s.sd, s.scene_of, s.xps, s.bounds, s.bounds + 1,
                                                   s.vals[buf], offs, list);
     else
-        k_neighbour_fill<false><<<g, 256, 0, st>>>(s.ed, s.sd, s.scene_of, s.xps, s.start, s.end,
+        k_neighbour_fill<false><<<g, 256, 0, st>>>(s.ed, s.sd, s.scene_of, s.xps, s.bounds, s.bounds + 1,
                                                    s.vals[buf], offs, list);
     SPH_LAUNCH_CHECK();
 }

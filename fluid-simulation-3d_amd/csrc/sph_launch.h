@@ -21,9 +21,16 @@ struct DevState {
     float4 *pos;  // x,y,z, w = 0.34 after an update (OutPositions layout, cc:107)
     float4 *vel;  // x,y,z, w unused
     float4 *xp;   // predicted position, w = bits of the exact u32 hash
-    // sort
+    // bucket sort: keys[0] bucket key and keys[1] ticket by particle id;
+    // vals[0] the buckets' slots in ticket order, vals[kSortedBuf] the
+    // particle id of each sorted slot
     uint32_t *keys[2];
     uint32_t *vals[2];
+    uint32_t *count;   // [table_capacity + scene capacity] bucket counts, then scene flags;
+                       // all zero between updates
+    uint32_t *parts;   // [count_blocks + 1] chunk sums, then the last-block counter
+    uint32_t count_blocks;
+    // radix sort of the Morton work order
     uint32_t *hist;        // [radix * radix_blocks] counts, then [radix] digit totals
     uint32_t radix_blocks;
     // sorted order
@@ -31,8 +38,7 @@ struct DevState {
     float4 *vs;    // velocity after stage A, sorted
     float4 *vs2;   // velocity after the pressure stage, sorted
     float4 *dens;  // rho, rho_near, RN(1/rho), RN(1/rho_near) (reciprocals 0 out of range)
-    uint32_t *start;  // [table_total] first sorted slot of each bucket
-    uint32_t *end;    // [table_total] one past the last slot (valid where start is)
+    uint32_t *bounds;  // [table_total + 1] bucket b holds sorted slots [bounds[b], bounds[b+1])
     // work order of the gather kernels
     uint32_t *wkeys[2];  // Morton ordering keys (sort buffers)
     uint32_t *wvals[2];  // particle ids (sort buffers); the sorted ones are the work order
@@ -40,6 +46,7 @@ struct DevState {
     uint32_t *rank;      // [capacity] sorted slot of each particle id
     float4 *colors;      // [capacity] speed colours, id order (null: not written)
     uint32_t *sflags;    // [scenes] grid key: 1 when a particle's cell lies outside the grid's box
+                         // (copied from the count flags by the bucket sort)
     uint32_t *nbr;       // [capacity * 64] neighbour rows by work index (nbr mode)
     uint32_t *ncnt;      // [capacity] neighbour count per row
     uint32_t capacity;       // particle slots allocated
@@ -89,16 +96,18 @@ inline LaunchCfg default_launch_cfg(bool grid)
     return sanitize_launch_cfg(c, grid);
 }
 
-// Stage A + B1: predict, cell, hash, key; vals = slot index.
+// Buffer pair index of the sorted particle ids (vals[kSortedBuf]).
+constexpr int kSortedBuf = 1;
+
+// Stage A + B1: predict, cell, hash, bucket key; counts each bucket's records.
 hipError_t launch_predict(const DevState &s, const LaunchCfg &c, hipStream_t st);
-// Bucket table clear to SPH_START_EMPTY.
-hipError_t launch_clear_table(const DevState &s, const LaunchCfg &c, hipStream_t st);
-// Stage B2: stable LSD radix sort of (keys, vals) over `bits` key bits.
-// Returns the index (0/1) of the buffer pair holding the result.
+// Stages B2 + B3: counting sort into (bucket, particle index) order, bucket
+// bounds, gather of xp, v into sorted order; clears the counts.
+hipError_t launch_bucket_sort(const DevState &s, const LaunchCfg &c, hipStream_t st);
+// Stable LSD radix sort of (keys, vals) over `bits` key bits (Morton work
+// order).  Returns the index (0/1) of the buffer pair holding the result.
 hipError_t launch_radix_sort(const DevState &s, uint32_t *const keys[2], uint32_t *const vals[2],
                              uint32_t bits, hipStream_t st, int *result_buf);
-// Stage B3 + gather into sorted order.
-hipError_t launch_reorder(const DevState &s, const LaunchCfg &c, int buf, hipStream_t st);
 // Stage C, D, E+F.
 hipError_t launch_density(const DevState &s, const LaunchCfg &c, int buf, hipStream_t st);
 hipError_t launch_pressure(const DevState &s, const LaunchCfg &c, int buf, hipStream_t st);
