@@ -265,11 +265,20 @@ int sph_download_colors(sph_handle h, float *rgba4)
 
 int sph_render_buffers(sph_handle h, const void **positions, const void **colors)
 {
+    SPH_GUARD_BEGIN
     SPH_NEED(h);
+    std::string err;
     const uint32_t off = h->eng->scene(0).p_off;
-    if (positions) *positions = h->eng->dev().pos + off;
+    int rc = SPH_OK;
+    if (positions) {
+        // kept in particle order from the first request on
+        const float4 *rp = h->eng->render_positions(&err);
+        if (!rp) rc = SPH_ERR_HIP;
+        *positions = rp ? rp + off : nullptr;
+    }
     if (colors) *colors = h->eng->colors_buffer() ? h->eng->colors_buffer() + off : nullptr;
-    return SPH_OK;
+    return done(rc, err);
+    SPH_GUARD_END
 }
 
 int sph_update(sph_handle h, float dt)

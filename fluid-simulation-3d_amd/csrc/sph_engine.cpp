@@ -274,6 +274,8 @@ int Engine::create(const std::vector<SceneSpec> &scenes, uint32_t capacity,
             rc = e->check(hipMemsetAsync(b, 0, sizeof(uint32_t) * capacity, e->stream_),
                           "clear work order", err);
     e->d_.work = e->d_.wvals[0];
+    // resident order: particle order
+    if (!rc) rc = e->check(launch_iota(e->d_.ids, capacity, 0, e->stream_), "resident ids", err);
     if (!rc) rc = e->check(hipStreamSynchronize(e->stream_), "sync", err);
     for (int i = 0; i < 7 && !rc; ++i) rc = e->check(hipEventCreate(&e->ev_[i]), "event", err);
     if (rc) {
@@ -309,15 +311,16 @@ int Engine::alloc(uint32_t cap, uint32_t tcap, std::string *err)
     if (!rc) rc = dm((void **)&d_scene_of_, u4, "scene map");
     if (!rc) rc = dm((void **)&d_.pos, f4, "pos");
     if (!rc) rc = dm((void **)&d_.vel, f4, "vel");
+    if (!rc) rc = dm((void **)&d_.ids, u4, "resident ids");
     if (!rc) rc = dm((void **)&d_.xp, f4, "xp");
+    if (!rc) rc = dm((void **)&d_.ps, f4, "ps");
     if (!rc) rc = dm((void **)&d_.xps, f4, "xps");
     if (!rc) rc = dm((void **)&d_.vs, f4, "vs");
     if (!rc) rc = dm((void **)&d_.vs2, f4, "vs2");
     if (!rc) rc = dm((void **)&d_.dens, f4, "dens");
-    for (int b = 0; b < 2 && !rc; ++b) {
-        rc = dm((void **)&d_.keys[b], u4, "keys");
-        if (!rc) rc = dm((void **)&d_.vals[b], u4, "vals");
-    }
+    for (int b = 0; b < 2 && !rc; ++b) rc = dm((void **)&d_.keys[b], u4, "keys");
+    if (!rc) rc = dm((void **)&d_.order, 2 * u4, "bucket order");
+    if (!rc) rc = dm((void **)&d_.sids, u4, "sorted ids");
     if (!rc) rc = dm((void **)&d_.hist, sizeof(uint32_t) * 256 * ((size_t)nblk + 1), "radix hist");
     if (!rc) rc = dm((void **)&d_.bounds, sizeof(uint32_t) * ((size_t)tcap + 1), "bucket bounds");
     const size_t count_bytes = sizeof(uint32_t) * ((size_t)tcap + scene_cap_);
@@ -348,7 +351,8 @@ Engine::~Engine()
     invalidate_graph();
     void *bufs[] = {d_ed_,       d_sd_,       d_scene_of_, d_.pos,     d_.vel,    d_.xp,
                     d_.xps,      d_.vs,       d_.vs2,      d_.dens,    d_.keys[0], d_.keys[1],
-                    d_.vals[0],  d_.vals[1],  d_.hist,     d_.bounds,  d_.count,  d_.parts,
+                    d_.ids,      d_.ps,       d_.order,    d_.sids,
+                    d_.hist,     d_.bounds,   d_.count,    d_.parts,
                     d_.wkeys[0], d_.wkeys[1], d_.wvals[0], d_.wvals[1], d_.rank,
                     d_.nbr,      d_.ncnt,     d_.sflags};
     for (void *b : bufs)
@@ -356,6 +360,7 @@ Engine::~Engine()
     if (tmp_pos_) (void)hipFree(tmp_pos_);
     if (tmp_vel_) (void)hipFree(tmp_vel_);
     if (colors_buf_) (void)hipFree(colors_buf_);
+    if (rpos_) (void)hipFree(rpos_);
     if (ed_h_) (void)hipHostFree(ed_h_);
     if (sd_pinned_) (void)hipHostFree(sd_pinned_);
     for (hipEvent_t &e : ev_)
@@ -417,12 +422,12 @@ int Engine::repack(const std::vector<RepackItem> &items, std::string *err)
         *err = "repack exceeds the engine's capacity";
         return SPH_ERR_INVALID;
     }
-    const size_t cap_bytes = sizeof(float4) * (size_t)d_.capacity;
-    if (!tmp_pos_) CK(hipMalloc((void **)&tmp_pos_, cap_bytes), "repack staging");
-    if (!tmp_vel_) CK(hipMalloc((void **)&tmp_vel_, cap_bytes), "repack staging");
-    device_bytes_ += 0;
-    // 1. moved scenes: old place -> staging; 2. staging -> new place; new
-    // scenes: host -> new place.  Scenes that keep their offset do not move.
+    int src = staging(err);
+    if (src) return src;
+    // 1. moved scenes: old place -> staging, in particle order; 2. staging ->
+    // new place; new scenes: host or generator -> new place.  Moved and new
+    // scenes start in particle order; scenes that keep their offset do not
+    // move.
     std::vector<uint32_t> new_off(items.size());
     uint32_t off = 0;
     for (size_t q = 0; q < items.size(); ++q) {
@@ -434,9 +439,7 @@ int Engine::repack(const std::vector<RepackItem> &items, std::string *err)
         const RepackItem &it = items[q];
         if (it.old_scene < 0 || sd_h_[it.old_scene].p_off == new_off[q]) continue;
         const uint32_t o = sd_h_[it.old_scene].p_off, n = it.spec.n;
-        CK(hipMemcpyAsync(tmp_pos_ + new_off[q], d_.pos + o, f4 * n, hipMemcpyDeviceToDevice, stream_),
-           "repack move");
-        CK(hipMemcpyAsync(tmp_vel_ + new_off[q], d_.vel + o, f4 * n, hipMemcpyDeviceToDevice, stream_),
+        CK(launch_unpermute_state(d_, o, n, tmp_pos_ + new_off[q], tmp_vel_ + new_off[q], stream_),
            "repack move");
     }
     for (size_t q = 0; q < items.size(); ++q) {
@@ -450,7 +453,10 @@ int Engine::repack(const std::vector<RepackItem> &items, std::string *err)
         } else if (sd_h_[it.old_scene].p_off != o) {
             CK(hipMemcpyAsync(d_.pos + o, tmp_pos_ + o, f4 * n, hipMemcpyDeviceToDevice, stream_), "repack move");
             CK(hipMemcpyAsync(d_.vel + o, tmp_vel_ + o, f4 * n, hipMemcpyDeviceToDevice, stream_), "repack move");
+        } else {
+            continue;  // in place, in its resident order
         }
+        CK(launch_iota(d_.ids + o, n, o, stream_), "repack order");
     }
     // descriptors of the new layout
     std::vector<SceneDesc> sd(items.size());
@@ -483,7 +489,53 @@ int Engine::repack(const std::vector<RepackItem> &items, std::string *err)
     if (rc) return rc;
     work_valid_ = false;
     have_tables_ = false;
+    for (uint32_t q = 0; q < n_scenes(); ++q) CK(refresh_render(q), "repack render positions");
     return SPH_OK;
+}
+
+int Engine::staging(std::string *err)
+{
+    const size_t cap_bytes = sizeof(float4) * (size_t)d_.capacity;
+    if (!tmp_pos_) CK(hipMalloc((void **)&tmp_pos_, cap_bytes), "staging");
+    if (!tmp_vel_) CK(hipMalloc((void **)&tmp_vel_, cap_bytes), "staging");
+    return SPH_OK;
+}
+
+hipError_t Engine::normalize_scene(uint32_t s)
+{
+    const SceneDesc &d = sd_h_[s];
+    const size_t bytes = sizeof(float4) * d.n;
+    hipError_t e = launch_unpermute_state(d_, d.p_off, d.n, tmp_pos_ + d.p_off, tmp_vel_ + d.p_off,
+                                          stream_);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_.pos + d.p_off, tmp_pos_ + d.p_off, bytes, hipMemcpyDeviceToDevice, stream_);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_.vel + d.p_off, tmp_vel_ + d.p_off, bytes, hipMemcpyDeviceToDevice, stream_);
+    if (e == hipSuccess) e = launch_iota(d_.ids + d.p_off, d.n, d.p_off, stream_);
+    return e;
+}
+
+hipError_t Engine::refresh_render(uint32_t s)
+{
+    if (!rpos_) return hipSuccess;
+    const SceneDesc &d = sd_h_[s];
+    return launch_unpermute_state(d_, d.p_off, d.n, rpos_ + d.p_off, nullptr, stream_);
+}
+
+const float4 *Engine::render_positions(std::string *err)
+{
+    if (!rpos_) {
+        if (check(hipMalloc((void **)&rpos_, sizeof(float4) * (size_t)d_.capacity), "render positions",
+                  err))
+            return nullptr;
+        device_bytes_ += sizeof(float4) * (size_t)d_.capacity;
+        for (uint32_t q = 0; q < n_scenes(); ++q)
+            if (check(refresh_render(q), "render positions", err)) return nullptr;
+        if (check(hipStreamSynchronize(stream_), "render positions", err)) return nullptr;
+        d_.rpos = rpos_;
+        invalidate_graph();
+    }
+    return rpos_;
 }
 
 int Engine::set_dt(float dt, std::string *err)
@@ -524,12 +576,22 @@ int Engine::upload(uint32_t s, const float *pos4, const float *vel4, std::string
     }
     const SceneDesc &d = sd_h_[s];
     const size_t bytes = sizeof(float4) * d.n;
+    if (!pos4 && !vel4) return SPH_OK;
+    if (!pos4 || !vel4) {
+        // the part not uploaded moves into particle order first
+        int rc = staging(err);
+        if (rc) return rc;
+        CK(normalize_scene(s), "upload reorder");
+    } else {
+        CK(launch_iota(d_.ids + d.p_off, d.n, d.p_off, stream_), "upload order");
+    }
     if (pos4)
         CK(hipMemcpyAsync(d_.pos + d.p_off, pos4, bytes, hipMemcpyHostToDevice, stream_),
            "upload positions");
     if (vel4)
         CK(hipMemcpyAsync(d_.vel + d.p_off, vel4, bytes, hipMemcpyHostToDevice, stream_),
            "upload velocities");
+    CK(refresh_render(s), "upload render positions");
     CK(hipStreamSynchronize(stream_), "sync after upload");
     return SPH_OK;
 }
@@ -580,6 +642,8 @@ int Engine::generate(uint32_t s, const GenPlan &g, std::string *err)
     }
     const SceneDesc &d = sd_h_[s];
     CK(launch_generate(g, d_.pos + d.p_off, d_.vel + d.p_off, stream_), "generate");
+    CK(launch_iota(d_.ids + d.p_off, d.n, d.p_off, stream_), "generate order");
+    CK(refresh_render(s), "generate render positions");
     CK(hipStreamSynchronize(stream_), "sync after generate");
     return SPH_OK;
 }
@@ -596,17 +660,17 @@ hipError_t Engine::enqueue_update(hipEvent_t *ev, bool rebuild_work)
     if (ev) LK(hipEventRecord(ev[1], stream_));
     if (rebuild_work) {
         int wb = 0;
-        LK(launch_radix_sort(d_, d_.wkeys, d_.wvals, 32, stream_, &wb));
+        LK(launch_radix_sort(d_, d_.wkeys, d_.wvals, d_.ids, 32, stream_, &wb));
         d_.work = d_.wvals[wb];
         work_valid_ = true;
     }
     LK(launch_bucket_sort(d_, cfg_, stream_));
     if (ev) LK(hipEventRecord(ev[2], stream_));
-    LK(launch_density(d_, cfg_, sorted_buf_, stream_));
+    LK(launch_density(d_, cfg_, stream_));
     if (ev) LK(hipEventRecord(ev[3], stream_));
-    LK(launch_pressure(d_, cfg_, sorted_buf_, stream_));
+    LK(launch_pressure(d_, cfg_, stream_));
     if (ev) LK(hipEventRecord(ev[4], stream_));
-    LK(launch_viscosity_integrate(d_, cfg_, sorted_buf_, stream_));
+    LK(launch_viscosity_integrate(d_, cfg_, stream_));
     if (ev) LK(hipEventRecord(ev[5], stream_));
 #undef LK
     return hipSuccess;
@@ -665,17 +729,24 @@ int Engine::autotune(float dt, int updates, double ms_out[3], std::string *err)
     }
     const uint32_t n = ed_h_->n_total;
     float4 *snap = nullptr;
-    CK(hipMalloc((void **)&snap, sizeof(float4) * 2 * (size_t)n), "autotune snapshot");
+    // positions, velocities and their resident order
+    CK(hipMalloc((void **)&snap, sizeof(float4) * 2 * (size_t)n + sizeof(uint32_t) * (size_t)n),
+       "autotune snapshot");
+    uint32_t *snap_ids = reinterpret_cast<uint32_t *>(snap + 2 * (size_t)n);
     auto save = [&]() -> hipError_t {
         hipError_t e = hipMemcpyAsync(snap, d_.pos, sizeof(float4) * n, hipMemcpyDeviceToDevice, stream_);
         if (e == hipSuccess)
             e = hipMemcpyAsync(snap + n, d_.vel, sizeof(float4) * n, hipMemcpyDeviceToDevice, stream_);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(snap_ids, d_.ids, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, stream_);
         return e;
     };
     auto restore = [&]() -> hipError_t {
         hipError_t e = hipMemcpyAsync(d_.pos, snap, sizeof(float4) * n, hipMemcpyDeviceToDevice, stream_);
         if (e == hipSuccess)
             e = hipMemcpyAsync(d_.vel, snap + n, sizeof(float4) * n, hipMemcpyDeviceToDevice, stream_);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_.ids, snap_ids, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, stream_);
         return e;
     };
     const LaunchCfg keep = cfg_;
@@ -714,6 +785,8 @@ int Engine::autotune(float dt, int updates, double ms_out[3], std::string *err)
         uint32_t used = 0;
         if (!rc) rc = profile_end(ms, &used, err);
         if (!rc) rc = check(restore(), "autotune restore", err);
+        for (uint32_t q = 0; q < n_scenes() && !rc; ++q)
+            rc = check(refresh_render(q), "autotune restore", err);
         if (rc) break;
         double *best = cands[ci].nbr ? row_ms : walk_ms;
         GatherVariant *pick = cands[ci].nbr ? row_v : walk_v;
@@ -839,12 +912,12 @@ int Engine::prime(std::string *err)
     CK(launch_predict(d_, cfg_, stream_), "prime predict");
     if (!grid_ && cfg_.needs_work()) {
         int wb = 0;
-        CK(launch_radix_sort(d_, d_.wkeys, d_.wvals, 32, stream_, &wb), "prime work order");
+        CK(launch_radix_sort(d_, d_.wkeys, d_.wvals, d_.ids, 32, stream_, &wb), "prime work order");
         d_.work = d_.wvals[wb];
         work_valid_ = true;
     }
     CK(launch_bucket_sort(d_, cfg_, stream_), "prime sort");
-    CK(launch_density(d_, cfg_, sorted_buf_, stream_), "prime density");
+    CK(launch_density(d_, cfg_, stream_), "prime density");
     CK(hipStreamSynchronize(stream_), "prime sync");
     have_tables_ = true;
     return SPH_OK;
@@ -858,12 +931,26 @@ int Engine::download(uint32_t s, float *pos4, float *vel4, float *dens2, std::st
     }
     const SceneDesc &d = sd_h_[s];
     CK(hipStreamSynchronize(stream_), "sync before download");
-    if (pos4)
-        CK(hipMemcpy(pos4, d_.pos + d.p_off, sizeof(float4) * d.n, hipMemcpyDeviceToHost),
-           "download positions");
-    if (vel4)
-        CK(hipMemcpy(vel4, d_.vel + d.p_off, sizeof(float4) * d.n, hipMemcpyDeviceToHost),
-           "download velocities");
+    if (pos4 || vel4) {
+        // resident order -> particle order on the host
+        std::vector<uint32_t> ids(d.n);
+        std::vector<float4> buf(d.n);
+        CK(hipMemcpy(ids.data(), d_.ids + d.p_off, sizeof(uint32_t) * d.n, hipMemcpyDeviceToHost),
+           "download order");
+        for (uint32_t k = 0; k < d.n; ++k)
+            if (ids[k] - d.p_off >= d.n) {
+                *err = "resident order is corrupt";
+                return SPH_ERR_STATE;
+            }
+        for (int a = 0; a < 2; ++a) {
+            float *out = a ? vel4 : pos4;
+            if (!out) continue;
+            CK(hipMemcpy(buf.data(), (a ? d_.vel : d_.pos) + d.p_off, sizeof(float4) * d.n,
+                         hipMemcpyDeviceToHost), a ? "download velocities" : "download positions");
+            float4 *o4 = reinterpret_cast<float4 *>(out);
+            for (uint32_t k = 0; k < d.n; ++k) o4[ids[k] - d.p_off] = buf[k];
+        }
+    }
     if (dens2) {
         if (!have_tables_) {
             memset(dens2, 0, sizeof(float) * 2 * d.n);
@@ -871,7 +958,7 @@ int Engine::download(uint32_t s, float *pos4, float *vel4, float *dens2, std::st
         }
         float2 *tmp = nullptr;
         CK(hipMalloc((void **)&tmp, sizeof(float2) * d_.capacity), "density staging");
-        hipError_t e1 = launch_unpermute_dens(d_, sorted_buf_, tmp, stream_);
+        hipError_t e1 = launch_unpermute_dens(d_, tmp, stream_);
         hipError_t e2 = e1 == hipSuccess ? hipStreamSynchronize(stream_) : e1;
         hipError_t e3 = e2 == hipSuccess ? hipMemcpy(dens2, tmp + d.p_off, sizeof(float2) * d.n,
                                                      hipMemcpyDeviceToHost)
@@ -894,35 +981,33 @@ int Engine::debug_spatial(uint32_t s, uint32_t *hash_id, uint32_t *key_id, uint3
         *err = "no update has run yet";
         return SPH_ERR_STATE;
     }
+    // everything comes from the sorted side of the last update: the particle
+    // index of each sorted slot, the exact hash in ps.w, the bucket bounds
     const SceneDesc &d = sd_h_[s];
     CK(hipStreamSynchronize(stream_), "sync");
-    std::vector<float4> xp(d.n);
-    std::vector<uint32_t> kid(d.n), si(d.n);
-    CK(hipMemcpy(xp.data(), d_.xp + d.p_off, sizeof(float4) * d.n, hipMemcpyDeviceToHost), "xp");
-    CK(hipMemcpy(kid.data(), d_.keys[0] + d.p_off, sizeof(uint32_t) * d.n,
-                 hipMemcpyDeviceToHost), "keys");
-    CK(hipMemcpy(si.data(), d_.vals[kSortedBuf] + d.p_off, sizeof(uint32_t) * d.n,
-                 hipMemcpyDeviceToHost), "sorted idx");
+    std::vector<float4> ps(d.n);
+    std::vector<uint32_t> si(d.n), bd((size_t)d.table + 1), sk(d.n, 0xffffffffu);
+    CK(hipMemcpy(ps.data(), d_.ps + d.p_off, sizeof(float4) * d.n, hipMemcpyDeviceToHost), "ps");
+    CK(hipMemcpy(si.data(), d_.sids + d.p_off, sizeof(uint32_t) * d.n, hipMemcpyDeviceToHost),
+       "sorted idx");
+    CK(hipMemcpy(bd.data(), d_.bounds + d.key_off, sizeof(uint32_t) * ((size_t)d.table + 1),
+                 hipMemcpyDeviceToHost), "bucket bounds");
+    for (uint32_t b = 0; b < d.table; ++b)
+        for (uint32_t k = bd[b]; k < bd[b + 1]; ++k)
+            if (k - d.p_off < d.n) sk[k - d.p_off] = b;
+    for (uint32_t k = 0; k < d.n; ++k) si[k] -= d.p_off;
     for (uint32_t k = 0; k < d.n; ++k) {
-        kid[k] -= d.key_off;
-        si[k] -= d.p_off;
+        if (si[k] >= d.n) continue;
+        if (hash_id) memcpy(&hash_id[si[k]], &ps[k].w, 4);
+        if (key_id) key_id[si[k]] = sk[k];
     }
-    if (hash_id)
-        for (uint32_t i = 0; i < d.n; ++i) memcpy(&hash_id[i], &xp[i].w, 4);
-    if (key_id) memcpy(key_id, kid.data(), sizeof(uint32_t) * d.n);
-    if (sorted_key)
-        for (uint32_t k = 0; k < d.n; ++k) sorted_key[k] = si[k] < d.n ? kid[si[k]] : 0xffffffffu;
+    if (sorted_key) memcpy(sorted_key, sk.data(), sizeof(uint32_t) * d.n);
     if (sorted_idx) memcpy(sorted_idx, si.data(), sizeof(uint32_t) * d.n);
-    if (start || end) {
-        // the reference's table: INT_MAX start for an empty bucket (cc:481)
-        std::vector<uint32_t> bd((size_t)d.table + 1);
-        CK(hipMemcpy(bd.data(), d_.bounds + d.key_off, sizeof(uint32_t) * ((size_t)d.table + 1),
-                     hipMemcpyDeviceToHost), "bucket bounds");
-        for (uint32_t b = 0; b < d.table; ++b) {
-            const bool empty = bd[b] == bd[b + 1];
-            if (start) start[b] = empty ? SPH_START_EMPTY : bd[b] - d.p_off;
-            if (end) end[b] = empty ? 0u : bd[b + 1] - d.p_off;
-        }
+    // the reference's table: INT_MAX start for an empty bucket (cc:481)
+    for (uint32_t b = 0; b < d.table; ++b) {
+        const bool empty = bd[b] == bd[b + 1];
+        if (start) start[b] = empty ? SPH_START_EMPTY : bd[b] - d.p_off;
+        if (end) end[b] = empty ? 0u : bd[b + 1] - d.p_off;
     }
     return SPH_OK;
 }
@@ -933,10 +1018,26 @@ int Engine::debug_predicted(uint32_t s, float *xp4, std::string *err)
         *err = "scene index out of range";
         return SPH_ERR_INVALID;
     }
+    if (!have_tables_) {
+        *err = "no update has run yet";
+        return SPH_ERR_STATE;
+    }
+    // the sorted copy of the last update's predicted positions
     const SceneDesc &d = sd_h_[s];
     CK(hipStreamSynchronize(stream_), "sync");
-    CK(hipMemcpy(xp4, d_.xp + d.p_off, sizeof(float4) * d.n, hipMemcpyDeviceToHost), "xp");
-    for (uint32_t i = 0; i < d.n; ++i) xp4[4 * i + 3] = 0.0f;
+    std::vector<float4> xps(d.n);
+    std::vector<uint32_t> si(d.n);
+    CK(hipMemcpy(xps.data(), d_.xps + d.p_off, sizeof(float4) * d.n, hipMemcpyDeviceToHost), "xps");
+    CK(hipMemcpy(si.data(), d_.sids + d.p_off, sizeof(uint32_t) * d.n, hipMemcpyDeviceToHost),
+       "sorted idx");
+    for (uint32_t k = 0; k < d.n; ++k) {
+        const uint32_t i = si[k] - d.p_off;
+        if (i >= d.n) continue;
+        xp4[4 * i + 0] = xps[k].x;
+        xp4[4 * i + 1] = xps[k].y;
+        xp4[4 * i + 2] = xps[k].z;
+        xp4[4 * i + 3] = 0.0f;
+    }
     return SPH_OK;
 }
 
@@ -962,7 +1063,7 @@ int Engine::debug_neighbours(uint32_t s, uint64_t *offs, uint32_t *list, uint64_
     do {
         if ((rc = check(hipMalloc((void **)&d_counts, sizeof(uint32_t) * n_all), "counts", err)))
             break;
-        if ((rc = check(launch_neighbour_count(d_, sorted_buf_, d_counts, stream_), "count", err)))
+        if ((rc = check(launch_neighbour_count(d_, d_counts, stream_), "count", err)))
             break;
         if ((rc = check(hipStreamSynchronize(stream_), "sync", err))) break;
         if ((rc = check(hipMemcpy(counts.data(), d_counts, sizeof(uint32_t) * n_all,
@@ -979,7 +1080,7 @@ int Engine::debug_neighbours(uint32_t s, uint64_t *offs, uint32_t *list, uint64_
         if ((rc = check(hipMemcpy(d_offs, offs_all.data(), sizeof(uint64_t) * (n_all + 1),
                                   hipMemcpyHostToDevice), "offs", err)))
             break;
-        if ((rc = check(launch_neighbour_fill(d_, sorted_buf_, d_offs, d_list, stream_), "fill", err)))
+        if ((rc = check(launch_neighbour_fill(d_, d_offs, d_list, stream_), "fill", err)))
             break;
         if ((rc = check(hipStreamSynchronize(stream_), "sync", err))) break;
         const uint64_t base = offs_all[d.p_off];

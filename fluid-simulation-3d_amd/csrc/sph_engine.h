@@ -71,10 +71,14 @@ class Engine {
     // Initial state of scene s generated on the device (g.n must equal its count).
     int generate(uint32_t s, const GenPlan &g, std::string *err);
     // Render hand-off: gradient stops (16 floats, null keeps them) and
-    // whether each update writes the speed colours (id order, RGBA).
+    // whether each update writes the speed colours (particle order, RGBA).
     int set_colors(const float *stops, bool enable, std::string *err);
     int download_colors(uint32_t s, float *rgba4, std::string *err);
     const float4 *colors_buffer() const { return colors_buf_; }
+    // Render positions (OutPositions, particle order): allocated and filled
+    // from the current state on the first call; every later update rewrites
+    // them.  Null on failure (err set).
+    const float4 *render_positions(std::string *err);
     // One Update(dt) for every scene (physicsWorld.cc:39-111).
     int update(float dt, std::string *err);
     // `substeps` Update(dt / substeps) calls; blocks until done when sync.
@@ -108,8 +112,8 @@ class Engine {
     }
     bool grid() const { return grid_; }
     // Times every gather-kernel variant on the current state (`updates`
-    // Update(dt) calls per candidate, from a snapshot of positions and
-    // velocities that is restored afterwards) and keeps the fastest variant of
+    // Update(dt) calls per candidate, from a snapshot of positions,
+    // velocities and their order that is restored afterwards) and keeps the fastest variant of
     // each kernel.  ms_out (may be null) gets the chosen variants' mean ms of
     // density, pressure, viscosity.  Densities and the debug tables are
     // recomputed by the next update.
@@ -119,7 +123,6 @@ class Engine {
     // Mean duration (ms) of each gather kernel over the last timed update.
     uint64_t device_bytes() const { return device_bytes_; }
     const DevState &dev() const { return d_; }
-    int sorted_buf() const { return sorted_buf_; }
     void set_work_period(uint32_t p) { work_period_ = p ? p : 1; }
 
   private:
@@ -128,6 +131,10 @@ class Engine {
     int push_descs(std::string *err);
     int set_dt(float dt, std::string *err);
     hipError_t enqueue_update(hipEvent_t *ev, bool rebuild_work);
+    // Resident slots of scene s into particle order (ids[k] = p_off + k).
+    hipError_t normalize_scene(uint32_t s);
+    // Render positions of scene s from its current state (if they exist).
+    hipError_t refresh_render(uint32_t s);
     bool rebuild_due() const
     {
         return !grid_ && cfg_.needs_work() && (!work_valid_ || work_updates_ % work_period_ == 0);
@@ -146,7 +153,6 @@ class Engine {
     uint32_t *d_scene_of_ = nullptr;
     uint32_t key_bits_ = 1;
     uint32_t scene_cap_ = 1;
-    int sorted_buf_ = kSortedBuf;
     bool have_tables_ = false;
     bool grid_ = false;  // cell-grid bucket key (else the reference's hash % table)
     bool use_graph_ = true;
@@ -161,8 +167,10 @@ class Engine {
     uint64_t device_bytes_ = 0;
     float dt_cached_ = -1.0f;
     float4 *colors_buf_ = nullptr;  // speed colours (allocated when first enabled)
-    float4 *tmp_pos_ = nullptr;  // repack staging
+    float4 *rpos_ = nullptr;        // render positions (allocated when first asked for)
+    float4 *tmp_pos_ = nullptr;  // repack / reorder staging
     float4 *tmp_vel_ = nullptr;
+    int staging(std::string *err);
     std::vector<hipEvent_t> prof_ev_;
     uint32_t prof_used_ = 0;
     bool prof_active_ = false;

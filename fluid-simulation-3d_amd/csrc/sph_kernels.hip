@@ -4,12 +4,12 @@
 // (FluidSimulation::Update, cc:39-111) and engine/physics/kernels.h.
 //
 // Pipeline of one Update(dt) over all scenes of an engine:
-//   k_predict          stage A + B1  (cc:42-48, 473-482), id order; counts
-//                      the records of each bucket
+//   k_predict          stage A + B1  (cc:42-48, 473-482) in resident order;
+//                      counts the records of each bucket
 //   k_count_reduce/scan/scatter/reorder  stages B2 + B3 (cc:481-496): a
 //                      counting sort over the buckets, ordered within a bucket
 //                      by particle index (the stable (key, index) order);
-//                      bucket bounds; gather of xp, v into sorted order
+//                      bucket bounds; gather of the state into sorted order
 //   k_density          stage C (cc:304-311, 325-365)
 //   k_pressure         stage D (cc:62-69, 367-422)
 //   k_viscosity_integrate  stage E (cc:71-78, 424-464, Jacobi) + F (cc:80-108)
@@ -24,7 +24,12 @@
 // Layout: SoA float4 arrays, one 16-byte load per particle per array, wave64
 // coalesced.  Gather kernels run one work-item per particle in SORTED order, so
 // the 64 lanes of a wave cover a handful of cells and read the same neighbour
-// buckets, which stay resident in the per-XCD L2.
+// buckets, which stay resident in the per-XCD L2.  The particle state (pos,
+// vel) stays in the sorted order of the last update (the resident order;
+// ids[k] is the particle index held by slot k): particles move little per
+// update, so the next sort's gathers and scatters are nearly sequential.
+// Results never depend on the resident order: the sort orders by (bucket,
+// particle index) whatever order its input is in.
 #include <hip/hip_runtime.h>
 
 #include "sph_device.h"
@@ -83,10 +88,12 @@ __device__ __forceinline__ uint32_t work_key(const EngineDesc &E, const SceneDes
 // ---------------------------------------------------------------------------
 // Stage A + B1
 // ---------------------------------------------------------------------------
-// count: [table_total] records per bucket (zero on entry; the atomic's old
-// value is the record's ticket, an arbitrary order within the bucket), then
-// [n_scenes] flags raised when a particle's cell lies outside the grid's box.
-// wkeys may be null (no Morton work order).
+// Runs over resident slots g.  count: [table_total] records per bucket (zero
+// on entry), then [n_scenes] flags raised when a particle's cell lies outside
+// the grid's box.  Each record takes a ticket, a distinct place in its bucket
+// in arbitrary order: one atomic per run of equal keys in consecutive lanes
+// (the resident order is nearly sorted, so a wave's 64 records fall in a few
+// runs).  wkeys may be null (no Morton work order).
 template <bool MULTI>
 __global__ void k_predict(const EngineDesc *__restrict__ ed, const SceneDesc *__restrict__ sd,
                           const uint32_t *__restrict__ scene_of, const float4 *__restrict__ pos,
@@ -95,35 +102,53 @@ __global__ void k_predict(const EngineDesc *__restrict__ ed, const SceneDesc *__
                           uint32_t *__restrict__ count, uint32_t *__restrict__ wkeys)
 {
     const EngineDesc E = *ed;
-    for (uint32_t g = global_id(); g < E.n_total; g += grid_stride()) {
-        const uint32_t s = MULTI ? scene_of[g] : 0u;
-        const SceneDesc &S = sd[s];
-        const float dt = S.dt;
-        float4 v = vel[g];
-        const float4 x = pos[g];
-        // cc:45: velocity += CalculateExternalFoce(...) * deltatime
-        v.x = v.x + S.g[0] * dt;
-        v.y = v.y + S.g[1] * dt;
-        v.z = v.z + S.g[2] * dt;
-        // cc:47: predictedPositions = positions + velocity * (1/120)
-        const float px = x.x + v.x * S.lookahead;
-        const float py = x.y + v.y * S.lookahead;
-        const float pz = x.z + v.z * S.lookahead;
-        const int cx = cell_coord(px, S.h), cy = cell_coord(py, S.h), cz = cell_coord(pz, S.h);
-        const uint32_t hash = hash_cell(cx, cy, cz);
-        vel[g] = v;
-        xp[g] = make_float4(px, py, pz, __uint_as_float(hash));
-        // bucket: GetKeyFromHash (cc:513-516), or the cell-grid key
-        const uint32_t key = S.key_off + (S.grid ? grid_key(cx, cy, cz, S.cell_lo, S.gdim, S.gmod)
-                                                 : fastmod(hash, S.mod_m, S.table));
-        keys[g] = key;
-        ticks[g] = atomicAdd(&count[key], 1u);
-        // grid key: a cell outside the grid's box wraps onto another cell's bucket
-        if (S.grid && ((uint32_t)cx - (uint32_t)S.cell_lo[0] >= S.gdim[0] ||
-                       (uint32_t)cy - (uint32_t)S.cell_lo[1] >= S.gdim[1] ||
-                       (uint32_t)cz - (uint32_t)S.cell_lo[2] >= S.gdim[2]))
-            count[E.table_total + S.index] = 1u;
-        if (wkeys) wkeys[g] = work_key(E, S, s, cx, cy, cz);
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t upto = (2ull << lane) - 1ull;  // lanes 0..lane
+    // whole waves iterate together (the ballots below need every lane)
+    for (uint32_t g0 = blockIdx.x * blockDim.x; g0 < E.n_total; g0 += grid_stride()) {
+        const uint32_t g = g0 + threadIdx.x;
+        const bool valid = g < E.n_total;
+        uint32_t key = 0xffffffffu;
+        if (valid) {
+            const uint32_t s = MULTI ? scene_of[g] : 0u;
+            const SceneDesc &S = sd[s];
+            const float dt = S.dt;
+            float4 v = vel[g];
+            const float4 x = pos[g];
+            // cc:45: velocity += CalculateExternalFoce(...) * deltatime
+            v.x = v.x + S.g[0] * dt;
+            v.y = v.y + S.g[1] * dt;
+            v.z = v.z + S.g[2] * dt;
+            // cc:47: predictedPositions = positions + velocity * (1/120)
+            const float px = x.x + v.x * S.lookahead;
+            const float py = x.y + v.y * S.lookahead;
+            const float pz = x.z + v.z * S.lookahead;
+            const int cx = cell_coord(px, S.h), cy = cell_coord(py, S.h), cz = cell_coord(pz, S.h);
+            const uint32_t hash = hash_cell(cx, cy, cz);
+            vel[g] = v;
+            xp[g] = make_float4(px, py, pz, __uint_as_float(hash));
+            // bucket: GetKeyFromHash (cc:513-516), or the cell-grid key
+            key = S.key_off + (S.grid ? grid_key(cx, cy, cz, S.cell_lo, S.gdim, S.gmod)
+                                      : fastmod(hash, S.mod_m, S.table));
+            keys[g] = key;
+            // grid key: a cell outside the grid's box wraps onto another cell's bucket
+            if (S.grid && ((uint32_t)cx - (uint32_t)S.cell_lo[0] >= S.gdim[0] ||
+                           (uint32_t)cy - (uint32_t)S.cell_lo[1] >= S.gdim[1] ||
+                           (uint32_t)cz - (uint32_t)S.cell_lo[2] >= S.gdim[2]))
+                count[E.table_total + S.index] = 1u;
+            if (wkeys) wkeys[g] = work_key(E, S, s, cx, cy, cz);
+        }
+        // runs of equal keys: a run starts at lane 0, after a different key,
+        // and after an invalid lane (only the last wave has any, at its end)
+        const uint32_t prev = __shfl_up(key, 1u, 64);
+        const uint64_t heads = __ballot(valid && (lane == 0u || prev != key)) | ~__ballot(valid);
+        const uint32_t first = 63u - (uint32_t)__builtin_clzll(heads & upto);  // this run's head
+        const uint64_t later = heads & ~upto;
+        const uint32_t len = (later ? (uint32_t)__builtin_ctzll(later) : 64u) - lane;
+        uint32_t base = 0;
+        if (valid && first == lane) base = atomicAdd(&count[key], len);
+        base = __shfl(base, (int)first, 64);
+        if (valid) ticks[g] = base + (lane - first);
     }
 }
 
@@ -385,45 +410,71 @@ __global__ __launch_bounds__(kSortBlock) void k_count_scan(const EngineDesc *__r
     if (blockIdx.x == 0 && tid == 0) bounds[T] = parts[gridDim.x];
 }
 
+// order[bounds[key] + ticket] = (resident slot, particle index)
 __global__ void k_count_scatter(const EngineDesc *__restrict__ ed,
                                 const uint32_t *__restrict__ keys,
                                 const uint32_t *__restrict__ ticks,
-                                const uint32_t *__restrict__ bounds, uint32_t *__restrict__ tmp)
+                                const uint32_t *__restrict__ ids,
+                                const uint32_t *__restrict__ bounds, uint2 *__restrict__ order)
 {
     const uint32_t n = ed->n_total;
     for (uint32_t g = global_id(); g < n; g += grid_stride()) {
         const uint32_t slot = bounds[keys[g]] + ticks[g];  // < n: the counts started at zero
-        if (slot < n) tmp[slot] = g;
+        if (slot < n) order[slot] = make_uint2(g, ids[g]);
     }
 }
 
+// A record's sorted slot is its bucket's first slot plus the number of the
+// bucket's records with a smaller particle index.  Writes the sorted copies
+// (xps with w = float(hash), the reference record's .y (cc:480); vs; ps with
+// w = the exact hash bits) and sids, the particle index of each sorted slot.
 __global__ void k_count_reorder(const EngineDesc *__restrict__ ed,
                                 const uint32_t *__restrict__ keys,
                                 const uint32_t *__restrict__ bounds,
-                                const uint32_t *__restrict__ tmp, uint32_t *__restrict__ svals,
+                                const uint2 *__restrict__ order, uint32_t *__restrict__ sids,
                                 const float4 *__restrict__ xp, const float4 *__restrict__ vel,
-                                float4 *__restrict__ xps, float4 *__restrict__ vs,
+                                const float4 *__restrict__ pos, float4 *__restrict__ xps,
+                                float4 *__restrict__ vs, float4 *__restrict__ ps,
                                 uint32_t *__restrict__ rank, uint32_t *__restrict__ count)
 {
     const uint32_t n = ed->n_total;
-    for (uint32_t k = global_id(); k < n; k += grid_stride()) {
-        const uint32_t g = tmp[k];
-        if (g >= n) continue;  // never taken while the counts start at zero
-        const uint32_t key = keys[g];
-        const uint32_t lo = bounds[key], hi = bounds[key + 1];
+    for (uint32_t t = global_id(); t < n; t += grid_stride()) {
+        const uint2 o = order[t];  // (resident slot, particle index)
+        if (o.x >= n) continue;   // never taken while the counts start at zero
+        const uint32_t key = keys[o.x];
+        const uint32_t lo = bounds[key], hi = min(bounds[key + 1], n);
         uint32_t before = 0;
-        for (uint32_t j = lo; j < hi; ++j) before += tmp[j] < g ? 1u : 0u;
+        for (uint32_t j = lo; j < hi; ++j) before += order[j].y < o.y ? 1u : 0u;
         const uint32_t slot = lo + before;
         if (slot >= hi) continue;  // as above
-        const float4 r = xp[g];
-        svals[slot] = g;
-        // the reference's lookup record keeps the hash as a float (cc:480)
+        const float4 r = xp[o.x], x0 = pos[o.x];
+        sids[slot] = o.y;
         xps[slot] = make_float4(r.x, r.y, r.z, (float)__float_as_uint(r.w));
-        vs[slot] = vel[g];
-        if (rank) rank[g] = slot;  // only the Morton work order reads it
+        vs[slot] = vel[o.x];
+        ps[slot] = make_float4(x0.x, x0.y, x0.z, r.w);
+        if (rank) rank[o.y] = slot;  // only the Morton work order reads it
     }
     const uint32_t nc = ed->table_total + ed->n_scenes;
     for (uint32_t i = global_id(); i < nc; i += grid_stride()) count[i] = 0u;
+}
+
+// Resident slots [0, n) of one scene into particle order:
+// dst_pos[ids[k] - id0] = pos[k] (and vel; either destination may be null).
+__global__ void k_unpermute_state(const float4 *__restrict__ pos, const float4 *__restrict__ vel,
+                                  const uint32_t *__restrict__ ids, uint32_t n, uint32_t id0,
+                                  float4 *__restrict__ dst_pos, float4 *__restrict__ dst_vel)
+{
+    for (uint32_t k = global_id(); k < n; k += grid_stride()) {
+        const uint32_t q = ids[k] - id0;
+        if (q >= n) continue;  // ids of the scene lie in [id0, id0 + n)
+        if (dst_pos) dst_pos[q] = pos[k];
+        if (dst_vel) dst_vel[q] = vel[k];
+    }
+}
+
+__global__ void k_iota(uint32_t *__restrict__ a, uint32_t n, uint32_t first)
+{
+    for (uint32_t k = global_id(); k < n; k += grid_stride()) a[k] = first + k;
 }
 
 // ---------------------------------------------------------------------------
@@ -1135,17 +1186,23 @@ __global__ __launch_bounds__(256) void k_pressure(SPH_GATHER_PARAMS, const float
     vs2[k] = v;
 }
 
+// Writes the new state at the sorted slot (pos, vel, ids: the next resident
+// order); rpos and colors (may be null) are the render buffers, in particle
+// order.
 template <bool MULTI, bool WORK, bool LIST, bool NBR, bool GRID>
 __global__ __launch_bounds__(256) void k_viscosity_integrate(SPH_GATHER_PARAMS,
                                                               const float4 *__restrict__ vs2,
+                                                              const float4 *__restrict__ ps,
                                                               float4 *__restrict__ pos,
                                                               float4 *__restrict__ vel,
+                                                              uint32_t *__restrict__ ids,
+                                                              float4 *__restrict__ rpos,
                                                               float4 *__restrict__ colors)
 {
     constexpr bool CONSUMER = NBR && GRID;  // reads the neighbour masks
     SPH_GATHER_PROLOGUE();
     const float4 vi = vs2[k];
-    const float4 x0 = pos[g];  // issued early: used after the gather
+    const float4 x0 = ps[k];  // issued early: used after the gather
     const float hh = S.hh, c6 = S.poly6c;
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     auto body = [&](uint32_t, float, float, float, float d2, const float4 &vj) {
@@ -1178,8 +1235,10 @@ __global__ __launch_bounds__(256) void k_viscosity_integrate(SPH_GATHER_PARAMS,
     if (S.half[1] - fabsf(x.y) <= 0.0f) { x.y = S.half[1] * sign_f(x.y); v.y *= S.damp_neg; }
     if (S.half[2] - fabsf(x.z) <= 0.0f) { x.z = S.half[2] * sign_f(x.z); v.z *= S.damp_neg; }
     x.w = 0.34f;  // OutPositions[i] = vec4(positions[i], 0.34f), cc:107
-    pos[g] = x;
-    vel[g] = v;
+    pos[k] = x;
+    vel[k] = v;
+    ids[k] = g;
+    if (rpos) rpos[g] = x;
     if (colors) {
         // render hand-off: the speed colour of the new velocity (fluidSimCPU.cc:100-125)
         float c[4];
@@ -1328,15 +1387,16 @@ hipError_t launch_predict(const DevState &s, const LaunchCfg &c, hipStream_t st)
 }
 
 hipError_t launch_radix_sort(const DevState &s, uint32_t *const keys[2], uint32_t *const vals[2],
-                             uint32_t bits, hipStream_t st, int *result_buf)
+                             const uint32_t *first_vals, uint32_t bits, hipStream_t st,
+                             int *result_buf)
 {
     int cur = 0;
     const uint32_t nblk = s.radix_blocks;
     for (uint32_t shift = 0; shift < bits; shift += kRadixBits) {
         k_radix_hist<<<nblk, kSortBlock, 0, st>>>(s.ed, keys[cur], shift, s.hist);
         k_radix_scan_digit<<<kRadix, kSortBlock, 0, st>>>(s.hist, nblk, s.hist + kRadix * nblk);
-        // first pass: values are the input slot indices, generated on the fly
-        k_radix_scatter<<<nblk, kSortBlock, 0, st>>>(s.ed, keys[cur], shift ? vals[cur] : nullptr,
+        // first pass: values from first_vals, or the input slot indices
+        k_radix_scatter<<<nblk, kSortBlock, 0, st>>>(s.ed, keys[cur], shift ? vals[cur] : first_vals,
                                                       keys[cur ^ 1],
                                                       vals[cur ^ 1], shift, s.hist,
                                                       s.hist + kRadix * nblk);
@@ -1351,10 +1411,10 @@ hipError_t launch_bucket_sort(const DevState &s, const LaunchCfg &c, hipStream_t
     const uint32_t nb = s.count_blocks, g = grid_for(s.capacity, c.block_stream);
     k_count_reduce<<<nb, kSortBlock, 0, st>>>(s.ed, s.count, s.parts, s.parts + nb + 1, s.sflags);
     k_count_scan<<<nb, kSortBlock, 0, st>>>(s.ed, s.count, s.parts, s.bounds);
-    k_count_scatter<<<g, c.block_stream, 0, st>>>(s.ed, s.keys[0], s.keys[1], s.bounds,
-                                                   s.vals[0]);
-    k_count_reorder<<<g, c.block_stream, 0, st>>>(s.ed, s.keys[0], s.bounds, s.vals[0],
-                                                   s.vals[kSortedBuf], s.xp, s.vel, s.xps, s.vs,
+    k_count_scatter<<<g, c.block_stream, 0, st>>>(s.ed, s.keys[0], s.keys[1], s.ids, s.bounds,
+                                                   s.order);
+    k_count_reorder<<<g, c.block_stream, 0, st>>>(s.ed, s.keys[0], s.bounds, s.order, s.sids,
+                                                   s.xp, s.vel, s.pos, s.xps, s.vs, s.ps,
                                                    s.grid ? nullptr : s.rank, s.count);
     SPH_LAUNCH_CHECK();
 }
@@ -1400,10 +1460,10 @@ inline uint32_t gather_grid(uint32_t cap, int block)
     } while (0)
 
 #define SPH_GATHER_ARGS                                                                \
-    s.ed, s.sd, s.scene_of, s.work, s.rank, s.vals[buf], s.xps, s.bounds, s.bounds + 1, s.nbr, s.ncnt,   \
+    s.ed, s.sd, s.scene_of, s.work, s.rank, s.sids, s.xps, s.bounds, s.bounds + 1, s.nbr, s.ncnt, \
         s.sflags
 
-hipError_t launch_density(const DevState &s, const LaunchCfg &c, int buf, hipStream_t st)
+hipError_t launch_density(const DevState &s, const LaunchCfg &c, hipStream_t st)
 {
     if (c.nbr && s.grid) {
         // the density pass records the neighbour masks
@@ -1434,49 +1494,62 @@ hipError_t launch_density(const DevState &s, const LaunchCfg &c, int buf, hipStr
     SPH_LAUNCH_CHECK();
 }
 
-hipError_t launch_pressure(const DevState &s, const LaunchCfg &c, int buf, hipStream_t st)
+hipError_t launch_pressure(const DevState &s, const LaunchCfg &c, hipStream_t st)
 {
     SPH_GATHER_LAUNCH(k_pressure, c.pressure, c.pressure.block, st, SPH_GATHER_ARGS, s.dens, s.vs,
                       s.vs2);
     SPH_LAUNCH_CHECK();
 }
 
-hipError_t launch_viscosity_integrate(const DevState &s, const LaunchCfg &c, int buf,
-                                      hipStream_t st)
+hipError_t launch_viscosity_integrate(const DevState &s, const LaunchCfg &c, hipStream_t st)
 {
     SPH_GATHER_LAUNCH(k_viscosity_integrate, c.viscosity, c.viscosity.block, st, SPH_GATHER_ARGS,
-                      s.vs2, s.pos, s.vel, s.colors);
+                      s.vs2, s.ps, s.pos, s.vel, s.ids, s.rpos, s.colors);
     SPH_LAUNCH_CHECK();
 }
 
-hipError_t launch_unpermute_dens(const DevState &s, int buf, float2 *out, hipStream_t st)
+hipError_t launch_unpermute_dens(const DevState &s, float2 *out, hipStream_t st)
 {
-    k_unpermute_dens<<<grid_for(s.capacity, 256), 256, 0, st>>>(s.ed, s.vals[buf], s.dens, out);
+    k_unpermute_dens<<<grid_for(s.capacity, 256), 256, 0, st>>>(s.ed, s.sids, s.dens, out);
     SPH_LAUNCH_CHECK();
 }
 
-hipError_t launch_neighbour_count(const DevState &s, int buf, uint32_t *counts, hipStream_t st)
+hipError_t launch_unpermute_state(const DevState &s, uint32_t off, uint32_t n, float4 *dst_pos,
+                                  float4 *dst_vel, hipStream_t st)
+{
+    k_unpermute_state<<<grid_for(n, 256), 256, 0, st>>>(s.pos + off, s.vel + off, s.ids + off, n,
+                                                        off, dst_pos, dst_vel);
+    SPH_LAUNCH_CHECK();
+}
+
+hipError_t launch_iota(uint32_t *a, uint32_t n, uint32_t first, hipStream_t st)
+{
+    k_iota<<<grid_for(n, 256), 256, 0, st>>>(a, n, first);
+    SPH_LAUNCH_CHECK();
+}
+
+hipError_t launch_neighbour_count(const DevState &s, uint32_t *counts, hipStream_t st)
 {
     const uint32_t g = grid_for(s.capacity, 256);
     if (s.multi)
         k_neighbour_count<true><<<g, 256, 0, st>>>(s.ed, s.sd, s.scene_of, s.xps, s.bounds, s.bounds + 1,
-                                                   s.vals[buf], counts);
+                                                   s.sids, counts);
     else
         k_neighbour_count<false><<<g, 256, 0, st>>>(s.ed, s.sd, s.scene_of, s.xps, s.bounds, s.bounds + 1,
-                                                    s.vals[buf], counts);
+                                                    s.sids, counts);
     SPH_LAUNCH_CHECK();
 }
 
-hipError_t launch_neighbour_fill(const DevState &s, int buf, const uint64_t *offs,
+hipError_t launch_neighbour_fill(const DevState &s, const uint64_t *offs,
                                  uint32_t *list, hipStream_t st)
 {
     const uint32_t g = grid_for(s.capacity, 256);
     if (s.multi)
         k_neighbour_fill<true><<<g, 256, 0, st>>>(s.ed, s.sd, s.scene_of, s.xps, s.bounds, s.bounds + 1,
-                                                  s.vals[buf], offs, list);
+                                                  s.sids, offs, list);
     else
         k_neighbour_fill<false><<<g, 256, 0, st>>>(s.ed, s.sd, s.scene_of, s.xps, s.bounds, s.bounds + 1,
-                                                   s.vals[buf], offs, list);
+                                                   s.sids, offs, list);
     SPH_LAUNCH_CHECK();
 }
 

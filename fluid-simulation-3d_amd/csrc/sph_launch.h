@@ -17,15 +17,17 @@ struct DevState {
     const uint32_t *scene_of;  // [n] scene of each slot (unused when single)
     bool multi;                // more than one scene
     bool grid;                 // cell-grid bucket key (SceneDesc::grid of every scene)
-    // particle state, id order
-    float4 *pos;  // x,y,z, w = 0.34 after an update (OutPositions layout, cc:107)
-    float4 *vel;  // x,y,z, w unused
-    float4 *xp;   // predicted position, w = bits of the exact u32 hash
-    // bucket sort: keys[0] bucket key and keys[1] ticket by particle id;
-    // vals[0] the buckets' slots in ticket order, vals[kSortedBuf] the
-    // particle id of each sorted slot
+    // particle state in the resident order: slot k holds particle ids[k]
+    // (the sorted order of the last update; any order gives the same results)
+    float4 *pos;    // x,y,z, w = 0.34 after an update (OutPositions layout, cc:107)
+    float4 *vel;    // x,y,z, w unused
+    uint32_t *ids;  // particle index (engine-wide) of each resident slot
+    float4 *xp;     // predicted position, w = bits of the exact u32 hash (resident order)
+    // bucket sort: keys[0] bucket key and keys[1] ticket by resident slot;
+    // order the (resident slot, particle index) pairs of each bucket in
+    // ticket order
     uint32_t *keys[2];
-    uint32_t *vals[2];
+    uint2 *order;
     uint32_t *count;   // [table_capacity + scene capacity] bucket counts, then scene flags;
                        // all zero between updates
     uint32_t *parts;   // [count_blocks + 1] chunk sums, then the last-block counter
@@ -38,13 +40,16 @@ struct DevState {
     float4 *vs;    // velocity after stage A, sorted
     float4 *vs2;   // velocity after the pressure stage, sorted
     float4 *dens;  // rho, rho_near, RN(1/rho), RN(1/rho_near) (reciprocals 0 out of range)
+    float4 *ps;    // position before the update, w = bits of the exact u32 hash
+    uint32_t *sids;  // particle index of each sorted slot
     uint32_t *bounds;  // [table_total + 1] bucket b holds sorted slots [bounds[b], bounds[b+1])
     // work order of the gather kernels
     uint32_t *wkeys[2];  // Morton ordering keys (sort buffers)
     uint32_t *wvals[2];  // particle ids (sort buffers); the sorted ones are the work order
     const uint32_t *work;  // = wvals[result] of the last rebuild
     uint32_t *rank;      // [capacity] sorted slot of each particle id
-    float4 *colors;      // [capacity] speed colours, id order (null: not written)
+    float4 *colors;      // [capacity] speed colours, particle order (null: not written)
+    float4 *rpos;        // [capacity] render positions, particle order (null: not written)
     uint32_t *sflags;    // [scenes] grid key: 1 when a particle's cell lies outside the grid's box
                          // (copied from the count flags by the bucket sort)
     uint32_t *nbr;       // [capacity * 64] neighbour rows by work index (nbr mode)
@@ -96,29 +101,33 @@ inline LaunchCfg default_launch_cfg(bool grid)
     return sanitize_launch_cfg(c, grid);
 }
 
-// Buffer pair index of the sorted particle ids (vals[kSortedBuf]).
-constexpr int kSortedBuf = 1;
-
 // Stage A + B1: predict, cell, hash, bucket key; counts each bucket's records.
 hipError_t launch_predict(const DevState &s, const LaunchCfg &c, hipStream_t st);
 // Stages B2 + B3: counting sort into (bucket, particle index) order, bucket
 // bounds, gather of xp, v into sorted order; clears the counts.
 hipError_t launch_bucket_sort(const DevState &s, const LaunchCfg &c, hipStream_t st);
 // Stable LSD radix sort of (keys, vals) over `bits` key bits (Morton work
-// order).  Returns the index (0/1) of the buffer pair holding the result.
+// order); the input values are first_vals (null: the input slot indices).
+// Returns the index (0/1) of the buffer pair holding the result.
 hipError_t launch_radix_sort(const DevState &s, uint32_t *const keys[2], uint32_t *const vals[2],
-                             uint32_t bits, hipStream_t st, int *result_buf);
+                             const uint32_t *first_vals, uint32_t bits, hipStream_t st,
+                             int *result_buf);
 // Stage C, D, E+F.
-hipError_t launch_density(const DevState &s, const LaunchCfg &c, int buf, hipStream_t st);
-hipError_t launch_pressure(const DevState &s, const LaunchCfg &c, int buf, hipStream_t st);
-hipError_t launch_viscosity_integrate(const DevState &s, const LaunchCfg &c, int buf,
-                                      hipStream_t st);
+hipError_t launch_density(const DevState &s, const LaunchCfg &c, hipStream_t st);
+hipError_t launch_pressure(const DevState &s, const LaunchCfg &c, hipStream_t st);
+hipError_t launch_viscosity_integrate(const DevState &s, const LaunchCfg &c, hipStream_t st);
 
-// Sorted -> id order densities (rho, rho_near) for download.
-hipError_t launch_unpermute_dens(const DevState &s, int buf, float2 *out, hipStream_t st);
-// Neighbour lists for parity checks (id order, walk order, self included).
-hipError_t launch_neighbour_count(const DevState &s, int buf, uint32_t *counts, hipStream_t st);
-hipError_t launch_neighbour_fill(const DevState &s, int buf, const uint64_t *offs,
+// Sorted -> particle order densities (rho, rho_near) for download.
+hipError_t launch_unpermute_dens(const DevState &s, float2 *out, hipStream_t st);
+// Resident slots [off, off + n) of one scene (particles off .. off + n - 1)
+// into particle order: dst_pos[i] / dst_vel[i] = state of particle off + i.
+hipError_t launch_unpermute_state(const DevState &s, uint32_t off, uint32_t n, float4 *dst_pos,
+                                  float4 *dst_vel, hipStream_t st);
+// a[k] = first + k for k < n.
+hipError_t launch_iota(uint32_t *a, uint32_t n, uint32_t first, hipStream_t st);
+// Neighbour lists for parity checks (particle order, walk order, self included).
+hipError_t launch_neighbour_count(const DevState &s, uint32_t *counts, hipStream_t st);
+hipError_t launch_neighbour_fill(const DevState &s, const uint64_t *offs,
                                  uint32_t *list, hipStream_t st);
 // Initial state of one scene from its plan (sph_generate.h), n = g.n slots
 // starting at pos / vel.

@@ -142,7 +142,10 @@ int sph_download_colors(sph_handle h, float *rgba4);
  * 0.34 -- OutPositions), colors = binding 1 ReadColors (vec4 RGBA; NULL
  * until sph_set_colors enables them).  n entries each, particle order,
  * valid until sph_destroy and rewritten by every update, so a renderer can
- * copy them into (or map them as) its SSBOs with no host round trip. */
+ * copy them into (or map them as) its SSBOs with no host round trip.  The
+ * simulation keeps its own state in a spatially sorted order; the position
+ * buffer is a particle-order copy, created and filled by the first call and
+ * written by every update from then on. */
 int sph_render_buffers(sph_handle h, const void **positions, const void **colors);
 /* Update(deltatime) (cc:39-111), blocking. */
 int sph_update(sph_handle h, float dt);
