@@ -91,7 +91,7 @@ int sph_debug_quotients(const float *a, const float *b, float *out, uint64_t n)
     if (n && (!a || !b || !out)) return fail(SPH_ERR_INVALID, "null argument");
     for (uint64_t i = 0; i < n; ++i) {
         const float y = sph::reciprocal_or_zero(b[i]);
-        out[i] = (y != 0.0f && sph::quotient_ok(a[i])) ? sph::quotient(a[i], b[i], y) : a[i] / b[i];
+        out[i] = (y != 0.0f && sph::quotients_ok(a[i])) ? sph::quotient(a[i], b[i], y) : a[i] / b[i];
     }
     return SPH_OK;
 }
@@ -107,6 +107,34 @@ int sph_device_count(void)
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;
     return n;
+}
+
+int sph_debug_reciprocal_mismatches(uint32_t lo_bits, uint32_t hi_bits, uint64_t *count,
+                                    uint32_t *first)
+{
+    SPH_GUARD_BEGIN
+    if (!count || lo_bits > hi_bits) return fail(SPH_ERR_INVALID, "null count or empty range");
+    if (!have_gpu()) return fail(SPH_ERR_HIP, "no HIP device visible");
+    struct Out {
+        unsigned long long bad;
+        uint32_t first;
+        uint32_t pad;
+    } host{0ull, 0xffffffffu, 0u};
+    Out *dev = nullptr;
+    hipStream_t st = nullptr;
+    hipError_t e = hipMalloc((void **)&dev, sizeof(Out));
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMemcpyAsync(dev, &host, sizeof(Out), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = sph::launch_recip_check(lo_bits, hi_bits, &dev->bad, &dev->first, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(&host, dev, sizeof(Out), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (st) (void)hipStreamDestroy(st);
+    if (dev) (void)hipFree(dev);
+    if (e != hipSuccess) return fail(SPH_ERR_HIP, std::string("reciprocal check: ") + hipGetErrorString(e));
+    *count = host.bad;
+    if (first) *first = host.first;
+    return SPH_OK;
+    SPH_GUARD_END
 }
 
 int sph_copy_calibration(uint64_t bytes, int reps, double *ms_per_copy)

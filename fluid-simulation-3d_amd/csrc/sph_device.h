@@ -98,6 +98,40 @@ SPH_HD bool quotient_ok(float a)
     return a == 0.0f || (m >= 0x1p-91f && m <= 0x1p100f);
 }
 
+// Sign-cleared bit pattern: non-negative floats order as these integers.
+SPH_HD uint32_t mag_bits(float x)
+{
+    uint32_t u;
+    __builtin_memcpy(&u, &x, sizeof u);
+    return u & 0x7fffffffu;
+}
+
+constexpr uint32_t kQuotLo = 0x12000000u;  // 2^-91
+constexpr uint32_t kQuotHi = 0x71800000u;  // 2^100
+
+// quotient_ok() of every argument, without branches: with u the magnitude
+// bits, a value fails when it is nonzero below the range (u - 1 < lo - 1,
+// unsigned, so zero wraps above) or above it (u > hi; infinities and NaNs
+// too).  One min and one max per value.
+template <typename... F>
+SPH_HD bool quotients_ok(F... a)
+{
+    uint32_t lo = 0xffffffffu, hi = 0u;
+    ((lo = (mag_bits(a) - 1u < lo ? mag_bits(a) - 1u : lo),
+      hi = (mag_bits(a) > hi ? mag_bits(a) : hi)),
+     ...);
+    return lo >= kQuotLo - 1u && hi <= kQuotHi;
+}
+
+// d in [2^-26, 2^26], the divisor range of quotient() (the bit pattern
+// itself: negative values, zeros and NaNs fall outside).
+SPH_HD bool divisor_ok(float d)
+{
+    uint32_t u;
+    __builtin_memcpy(&u, &d, sizeof u);
+    return u - 0x32800000u <= 0x4c800000u - 0x32800000u;
+}
+
 SPH_HD float quotient(float a, float b, float y)
 {
     const float q0 = a * y;
@@ -105,10 +139,37 @@ SPH_HD float quotient(float a, float b, float y)
     return fmaf(r, y, q0);
 }
 
-// y = RN(1/x) for x in [2^-26, 2^26], else 0 (no fast quotient by x).
-SPH_HD float reciprocal_or_zero(float x)
+// RN(1/x) for x in [2^-26, 2^26] (divisor_ok).  On the GPU: the hardware
+// reciprocal estimate y0 (within one ulp) refined by one Newton-Raphson step
+// with fused multiply-adds, e = 1 - x*y0 (exact) and y0 + e*y0 (Markstein's
+// correctly rounded reciprocal, IBM J. Res. Dev. 34(1), 1990); it equals IEEE
+// 1/x for every float of the range (checked exhaustively on the GPU by
+// sph_debug_reciprocal_mismatches).  On the host: IEEE 1/x.
+SPH_HD float recip_rn(float x)
 {
-    return (x >= 0x1p-26f && x <= 0x1p26f) ? 1.0f / x : 0.0f;
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float y0 = __builtin_amdgcn_rcpf(x);
+    const float e = fmaf(-x, y0, 1.0f);
+    return fmaf(e, y0, y0);
+#else
+    return 1.0f / x;
+#endif
+}
+
+// y = RN(1/x) for x in [2^-26, 2^26], else 0 (no fast quotient by x).
+SPH_HD float reciprocal_or_zero(float x) { return divisor_ok(x) ? recip_rn(x) : 0.0f; }
+
+// True when |x|, |y|, |z| >= 2^-60.  Then every nonzero difference between
+// this point's coordinates and another float's has magnitude >= 2^-84:
+// floats of magnitude >= 2^-61 are multiples of 2^-84, and a float below
+// 2^-61 lies at least 2^-61 away.  Such differences are valid quotient()
+// numerators whenever the divisor d bounds them (|difference| <= d(1+2^-22)
+// for the components of a distance d <= 2^26).
+SPH_HD bool coords_clear_of_zero(float x, float y, float z)
+{
+    const uint32_t a = mag_bits(x), b = mag_bits(y), c = mag_bits(z);
+    const uint32_t m = a < b ? (a < c ? a : c) : (b < c ? b : c);
+    return m >= 0x21800000u;  // 2^-60
 }
 
 // glm::sign for float (func_common.inl:144-150): (0 < x) - (x < 0).

@@ -1131,10 +1131,15 @@ __global__ __launch_bounds__(256) void k_pressure(SPH_GATHER_PARAMS, const float
     const float4 di = dens[k];
     const float h = S.h, dw2c = S.dw2c, dw3c = S.dw3c, rho0 = S.rho0, kp = S.k, kn = S.k_near;
     const float P = (di.x - rho0) * kp, Pn = di.y * kn;  // cc:371-372
+    // the quotients by d need only d's range when the coordinate differences
+    // to every neighbour are either zero or of magnitude >= 2^-84
+    const int near_ok = (int)coords_clear_of_zero(pi.x, pi.y, pi.z);
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    // dj = (rho_j, rho_near_j, RN(1/rho_j), RN(1/rho_near_j)); the six
-    // divisions by rho_j / rho_near_j use those reciprocals (quotient(), exact
-    // in its stated range) and fall back to IEEE division outside it
+    // Every division is correctly rounded.  dj = (rho_j, rho_near_j,
+    // RN(1/rho_j), RN(1/rho_near_j)); the six divisions by rho_j / rho_near_j
+    // use those reciprocals and the three by d use RN(1/d), through
+    // quotient() (exact in its stated range), and fall back to IEEE division
+    // outside that range
     auto body = [&](uint32_t, float ox, float oy, float oz, float d2, const float4 &dj) {
         const float pj = (dj.x - rho0) * kp, pnj = dj.y * kn;  // cc:407-408
         const float sp = (P + pj) * 0.5f;     // cc:410
@@ -1142,18 +1147,24 @@ __global__ __launch_bounds__(256) void k_pressure(SPH_GATHER_PARAMS, const float
         const float d = sqrtf(d2);
         float dirx = 0.0f, diry = 1.0f, dirz = 0.0f;  // cc:414
         if (d > 0.0f) {
-            dirx = ox / d;
-            diry = oy / d;
-            dirz = oz / d;
+            if ((near_ok & (int)divisor_ok(d)) != 0) {
+                const float yd = recip_rn(d);
+                dirx = quotient(ox, d, yd);
+                diry = quotient(oy, d, yd);
+                dirz = quotient(oz, d, yd);
+            } else {
+                dirx = ox / d;
+                diry = oy / d;
+                dirz = oz / d;
+            }
         }
         const float v = h - d;
         const float a = -v * dw2c;      // SmoothingDerivativePow2, kernels.h:55
         const float b = -v * v * dw3c;  // SmoothingDerivativePow3, kernels.h:67
         const float tx = dirx * a * sp, ty = diry * a * sp, tz = dirz * a * sp;
         const float ux = dirx * b * snp, uy = diry * b * snp, uz = dirz * b * snp;
-        const bool fast = ((int)(dj.z != 0.0f) & (int)(dj.w != 0.0f) & (int)quotient_ok(tx) &
-                           (int)quotient_ok(ty) & (int)quotient_ok(tz) & (int)quotient_ok(ux) &
-                           (int)quotient_ok(uy) & (int)quotient_ok(uz)) != 0;
+        const bool fast = ((int)(dj.z != 0.0f) & (int)(dj.w != 0.0f) &
+                           (int)quotients_ok(tx, ty, tz, ux, uy, uz)) != 0;
         if (fast) {
             fx += quotient(tx, dj.x, dj.z);  // cc:416
             fy += quotient(ty, dj.x, dj.z);
@@ -1338,6 +1349,22 @@ __global__ void k_generate(const GenPlan g, float4 *__restrict__ pos, float4 *__
         pos[q] = make_float4(p[0], p[1], p[2], p[3]);
         vel[q] = make_float4(v[0], v[1], v[2], v[3]);
     }
+}
+
+// Floats with bit patterns in [lo, hi] whose recip_rn differs from IEEE 1/x:
+// their count and the smallest such pattern.
+__global__ void k_recip_check(uint32_t lo, uint32_t hi, unsigned long long *__restrict__ bad,
+                              uint32_t *__restrict__ first_bad)
+{
+    uint32_t c = 0;
+    for (uint64_t u = (uint64_t)lo + global_id(); u <= hi; u += grid_stride()) {
+        const float x = __uint_as_float((uint32_t)u);
+        if (__float_as_uint(recip_rn(x)) != __float_as_uint(1.0f / x)) {
+            ++c;
+            atomicMin(first_bad, (uint32_t)u);
+        }
+    }
+    if (c) atomicAdd(bad, (unsigned long long)c);
 }
 
 // Plain streaming copy: the reference point for HBM byte counters.
@@ -1556,6 +1583,13 @@ hipError_t launch_neighbour_fill(const DevState &s, const uint64_t *offs,
 hipError_t launch_generate(const GenPlan &g, float4 *pos, float4 *vel, hipStream_t st)
 {
     k_generate<<<grid_for(g.n, 256), 256, 0, st>>>(g, pos, vel);
+    SPH_LAUNCH_CHECK();
+}
+
+hipError_t launch_recip_check(uint32_t lo, uint32_t hi, unsigned long long *bad,
+                              uint32_t *first_bad, hipStream_t st)
+{
+    k_recip_check<<<4096, 256, 0, st>>>(lo, hi, bad, first_bad);
     SPH_LAUNCH_CHECK();
 }
 

@@ -132,6 +132,11 @@ hipError_t launch_neighbour_fill(const DevState &s, const uint64_t *offs,
 // Initial state of one scene from its plan (sph_generate.h), n = g.n slots
 // starting at pos / vel.
 hipError_t launch_generate(const GenPlan &g, float4 *pos, float4 *vel, hipStream_t st);
+// Count of floats with bit patterns in [lo, hi] whose kernel reciprocal
+// (recip_rn) differs from IEEE 1/x, and the smallest such pattern (atomic
+// min into *first_bad, which the caller sets first).
+hipError_t launch_recip_check(uint32_t lo, uint32_t hi, unsigned long long *bad,
+                              uint32_t *first_bad, hipStream_t st);
 // Plain copy of n float4 (HBM counter calibration).
 hipError_t launch_copy(const float4 *src, float4 *dst, size_t n, hipStream_t st);
 // Count of non-finite position/velocity components (error flag).

@@ -180,6 +180,7 @@ def lib():
         "sph_hash_cell": (u32, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
         "sph_bucket_of": (u32, [u32, u32]),
         "sph_debug_quotients": (i32, [p, p, p, u64]),
+        "sph_debug_reciprocal_mismatches": (i32, [u32, u32, p, p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -236,6 +237,14 @@ def copy_calibration(nbytes, reps=3):
 def hash_cell(cx, cy, cz):
     """HashCell (physicsWorld.cc:505-511) as the kernels compute it."""
     return lib().sph_hash_cell(int(cx), int(cy), int(cz))
+
+
+def debug_reciprocal_mismatches(lo_bits, hi_bits):
+    """(count, first) of floats in the bit range whose GPU reciprocal differs from IEEE 1/x."""
+    cnt = np.zeros(1, np.uint64)
+    first = np.zeros(1, np.uint32)
+    _check(lib().sph_debug_reciprocal_mismatches(int(lo_bits), int(hi_bits), _ptr(cnt), _ptr(first)))
+    return int(cnt[0]), int(first[0])
 
 
 def debug_quotients(a, b):

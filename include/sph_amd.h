@@ -224,6 +224,13 @@ uint32_t sph_bucket_of(uint32_t hash, uint32_t table_size);
  * its range conditions hold, IEEE division elsewhere), on the host from the
  * same source.  Equal to IEEE a / b except that -0 / b gives +0. */
 int sph_debug_quotients(const float *a, const float *b, float *out, uint64_t n);
+/* Parity support, no reference counterpart: on the GPU, counts the floats x
+ * with bit patterns in [lo_bits, hi_bits] whose kernel reciprocal (the
+ * estimate-and-correction RN(1/x) behind the pressure kernel's quotients)
+ * differs from IEEE 1/x; *first (may be NULL) gets the smallest such bit
+ * pattern, 0xffffffff when there is none. */
+int sph_debug_reciprocal_mismatches(uint32_t lo_bits, uint32_t hi_bits, uint64_t *count,
+                                    uint32_t *first);
 
 /* ---- batching: many independent scenes per launch ----
  * No reference counterpart (the reference simulates one scene per process,

@@ -117,6 +117,20 @@ def test_kernel_quotient_equals_ieee_division():
         want = (a / b).astype(np.float32)
     got = S.debug_quotients(a, b)
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    # the edges of the fast range and the values outside it (tiny, huge,
+    # subnormal, infinite, NaN numerators; divisors at and beyond 2^-26, 2^26)
+    def bits(u):
+        return np.array(u, np.uint32).view(np.float32)
+    edge_a = bits([0x12000000, 0x11FFFFFF, 0x71800000, 0x71800001, 0x00000001, 0x007FFFFF,
+                   0x7F800000, 0x7FC00000, 0x92000000, 0xF1800000, 0x3F800000])
+    edge_b = bits([0x32800000, 0x327FFFFF, 0x4C800000, 0x4C800001, 0x3F800000, 0x3FFFFFFF])
+    a, b = np.meshgrid(edge_a, edge_b)
+    a, b = a.ravel(), b.ravel()
+    with np.errstate(over="ignore", under="ignore", invalid="ignore"):
+        want = (a / b).astype(np.float32)
+    got = S.debug_quotients(a, b)
+    same = (got.view(np.uint32) == want.view(np.uint32)) | (np.isnan(got) & np.isnan(want))
+    assert same.all(), list(zip(a[~same], b[~same]))
     # zeros: +0 exact; -0 gives +0 (a zero term leaves a sum from +0 unchanged)
     z = S.debug_quotients(np.array([0.0, -0.0], np.float32), np.array([3.0, 3.0], np.float32))
     assert z[0] == 0 and not np.signbit(z[0]) and z[1] == 0

@@ -76,3 +76,29 @@ def test_batch_of_one_particle_scenes():
         assert_bitexact(got[s][0][:, :3], po[:, :3], f"scene {s} position")
         assert_bitexact(got[s][1][:, :3], vo[:, :3], f"scene {s} velocity")
         assert_bitexact(got[s][2], do, f"scene {s} density")
+
+
+def test_coordinates_near_zero_and_near_coincident_pairs():
+    """Zero, tiny and subnormal coordinates (differences below 2^-84) and
+    pairs closer than 2^-26 take the pressure pass's IEEE divisions by d;
+    every other pair its reciprocal quotients.  Both stay bit-exact."""
+    n = 600
+    rng = np.random.default_rng(5)
+    p = W.scene_params(h=0.2, box=(2.0, 2.0, 2.0))
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, 3] = np.float32(0.34)
+    pos[:, :3] = rng.uniform(-0.15, 0.15, (n, 3)).astype(np.float32)
+    pos[:100, 0] = rng.choice(np.array([0.0, 1e-30, -3e-35, 1e-38, 1.4e-45], np.float32), 100)
+    pos[100:200, 1] = rng.choice(np.array([0.0, 1e-25, -1e-29], np.float32), 100)
+    # one float32 ulp apart near 0.1 (about 7.5e-9 < 2^-26)
+    pos[200:300, :3] = np.nextafter(pos[300:400, :3], np.float32(1.0))
+    vel = np.zeros((n, 4), np.float32)
+    _check(p, pos, vel, "near zero")
+
+
+def test_reciprocal_equals_ieee_on_the_whole_divisor_range():
+    """The pressure kernel's RN(1/d) (hardware estimate + one Newton step) is
+    IEEE 1/d for every float d in [2^-26, 2^26]: 436M values on the GPU."""
+    announce("reciprocal sweep")
+    count, first = S.debug_reciprocal_mismatches(0x32800000, 0x4C800000)
+    assert count == 0, f"{count} mismatches, first at bits {first:#010x}"
