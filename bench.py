@@ -31,11 +31,11 @@ from sph_amd import workloads as W  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 # Algorithmic HBM bytes per particle per launch of each timed stage: each
 # array read or written once, neighbour re-reads counted as on-chip.
-#   predict:   R pos, vel (16+16), W vel, xp (16+16), W key, slot (4+4)
-#   density:   R xp (16), W rho, rho_n, P, P_n (16); bucket table reads ~0
+#   predict:   R pos, vel (16+16), W vel, xp (16+16), W key, ticket (4+4)
+#   density:   R xp (16), W rho, rho_n, 1/rho, 1/rho_n (16); bucket table reads ~0
 #   pressure:  R xp, dens, v (48), W v (16)
-#   viscosity+integrate: R xp, v, slot (36), R+W pos (32), W vel (16)
-ALG_BYTES = {"predict": 72.0, "density": 32.0, "pressure": 64.0, "viscosity_integrate": 84.0}
+#   viscosity+integrate: R xp, v, pos, id (52), W pos, vel, id (36)
+ALG_BYTES = {"predict": 72.0, "density": 32.0, "pressure": 64.0, "viscosity_integrate": 88.0}
 STAGES = ["predict", "spatial", "density", "pressure", "viscosity_integrate"]
 # Per-dispatch HBM traffic of each kernel instantiation on uniform_1m, from two
 # rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of this script, each scaled by
