@@ -1037,8 +1037,12 @@ __device__ __forceinline__ void gather_row(const SceneDesc &S, const float4 *__r
     }
 }
 
+// The second __launch_bounds__ argument is the minimum waves per SIMD the
+// register allocation must allow: the single-scene density pass fits 64
+// VGPRs (8 waves) and the viscosity pass 96 (5 waves) without spilling; the
+// multi-scene forms keep the occupancy they reach unconstrained.
 template <bool MULTI, bool WORK, bool LIST, bool NBR, bool GRID>
-__global__ __launch_bounds__(256) void k_density(SPH_GATHER_PARAMS, float4 *__restrict__ dens)
+__global__ __launch_bounds__(256, MULTI ? 6 : 8) void k_density(SPH_GATHER_PARAMS, float4 *__restrict__ dens)
 {
     constexpr bool CONSUMER = false;
     SPH_GATHER_PROLOGUE();
@@ -1201,7 +1205,7 @@ __global__ __launch_bounds__(256) void k_pressure(SPH_GATHER_PARAMS, const float
 // order); rpos and colors (may be null) are the render buffers, in particle
 // order.
 template <bool MULTI, bool WORK, bool LIST, bool NBR, bool GRID>
-__global__ __launch_bounds__(256) void k_viscosity_integrate(SPH_GATHER_PARAMS,
+__global__ __launch_bounds__(256, MULTI ? 4 : 5) void k_viscosity_integrate(SPH_GATHER_PARAMS,
                                                               const float4 *__restrict__ vs2,
                                                               const float4 *__restrict__ ps,
                                                               float4 *__restrict__ pos,
