@@ -312,7 +312,6 @@ int Engine::alloc(uint32_t cap, uint32_t tcap, std::string *err)
     if (!rc) rc = dm((void **)&d_.pos, f4, "pos");
     if (!rc) rc = dm((void **)&d_.vel, f4, "vel");
     if (!rc) rc = dm((void **)&d_.ids, u4, "resident ids");
-    if (!rc) rc = dm((void **)&d_.xp, f4, "xp");
     if (!rc) rc = dm((void **)&d_.ps, f4, "ps");
     if (!rc) rc = dm((void **)&d_.xps, f4, "xps");
     if (!rc) rc = dm((void **)&d_.vs, f4, "vs");
@@ -323,7 +322,8 @@ int Engine::alloc(uint32_t cap, uint32_t tcap, std::string *err)
     if (!rc) rc = dm((void **)&d_.sids, u4, "sorted ids");
     if (!rc) rc = dm((void **)&d_.hist, sizeof(uint32_t) * 256 * ((size_t)nblk + 1), "radix hist");
     if (!rc) rc = dm((void **)&d_.bounds, sizeof(uint32_t) * ((size_t)tcap + 1), "bucket bounds");
-    const size_t count_bytes = sizeof(uint32_t) * ((size_t)tcap + scene_cap_);
+    d_.count_size = tcap + scene_cap_;
+    const size_t count_bytes = sizeof(uint32_t) * (size_t)d_.count_size;
     if (!rc) rc = dm((void **)&d_.count, count_bytes, "bucket counts");
     if (!rc) rc = check(hipMemset(d_.count, 0, count_bytes), "bucket counts", err);
     if (!rc) rc = dm((void **)&d_.parts, sizeof(uint32_t) * ((size_t)cblk + 2), "count chunks");
@@ -349,7 +349,7 @@ Engine::~Engine()
 {
     if (stream_) (void)hipStreamSynchronize(stream_);
     invalidate_graph();
-    void *bufs[] = {d_ed_,       d_sd_,       d_scene_of_, d_.pos,     d_.vel,    d_.xp,
+    void *bufs[] = {d_ed_,       d_sd_,       d_scene_of_, d_.pos,     d_.vel,
                     d_.xps,      d_.vs,       d_.vs2,      d_.dens,    d_.keys[0], d_.keys[1],
                     d_.ids,      d_.ps,       d_.order,    d_.sids,
                     d_.hist,     d_.bounds,   d_.count,    d_.parts,
@@ -370,7 +370,7 @@ Engine::~Engine()
 
 void Engine::invalidate_graph()
 {
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < 4; ++i) {
         if (graph_exec_[i]) (void)hipGraphExecDestroy(graph_exec_[i]);
         if (graph_[i]) (void)hipGraphDestroy(graph_[i]);
         graph_exec_[i] = nullptr;
@@ -424,6 +424,7 @@ int Engine::repack(const std::vector<RepackItem> &items, std::string *err)
     }
     int src = staging(err);
     if (src) return src;
+    recorded_ = false;
     // 1. moved scenes: old place -> staging, in particle order; 2. staging ->
     // new place; new scenes: host or generator -> new place.  Moved and new
     // scenes start in particle order; scenes that keep their offset do not
@@ -541,6 +542,7 @@ const float4 *Engine::render_positions(std::string *err)
 int Engine::set_dt(float dt, std::string *err)
 {
     if (dt == dt_cached_) return SPH_OK;
+    recorded_ = false;  // recorded with the previous dt
     for (SceneDesc &d : sd_h_) d.dt = dt;
     dt_cached_ = dt;
     return push_descs(err);
@@ -556,6 +558,7 @@ int Engine::set_params(uint32_t s, const sph_params &p, std::string *err)
         *err = m;
         return SPH_ERR_INVALID;
     }
+    recorded_ = false;
     SceneDesc &d = sd_h_[s];
     const float dt = d.dt;
     const uint32_t p_off = d.p_off, k_off = d.key_off;
@@ -577,6 +580,7 @@ int Engine::upload(uint32_t s, const float *pos4, const float *vel4, std::string
     const SceneDesc &d = sd_h_[s];
     const size_t bytes = sizeof(float4) * d.n;
     if (!pos4 && !vel4) return SPH_OK;
+    recorded_ = false;
     if (!pos4 || !vel4) {
         // the part not uploaded moves into particle order first
         int rc = staging(err);
@@ -641,6 +645,7 @@ int Engine::generate(uint32_t s, const GenPlan &g, std::string *err)
         return SPH_ERR_INVALID;
     }
     const SceneDesc &d = sd_h_[s];
+    recorded_ = false;
     CK(launch_generate(g, d_.pos + d.p_off, d_.vel + d.p_off, stream_), "generate");
     CK(launch_iota(d_.ids + d.p_off, d.n, d.p_off, stream_), "generate order");
     CK(refresh_render(s), "generate render positions");
@@ -648,15 +653,16 @@ int Engine::generate(uint32_t s, const GenPlan &g, std::string *err)
     return SPH_OK;
 }
 
-hipError_t Engine::enqueue_update(hipEvent_t *ev, bool rebuild_work)
+hipError_t Engine::enqueue_update(hipEvent_t *ev, bool rebuild_work, bool fresh)
 {
     // ev (optional): 6 events bracketing predict | spatial (+ work order) |
-    // density | pressure | viscosity+integrate
+    // density | pressure | viscosity+integrate.  fresh: the previous update
+    // did not record this one's buckets (its viscosity pass does otherwise)
     hipError_t e;
 #define LK(x)                              \
     if ((e = (x)) != hipSuccess) return e;
     if (ev) LK(hipEventRecord(ev[0], stream_));
-    LK(launch_predict(d_, cfg_, stream_));
+    if (fresh) LK(launch_predict(d_, cfg_, stream_));
     if (ev) LK(hipEventRecord(ev[1], stream_));
     if (rebuild_work) {
         int wb = 0;
@@ -685,11 +691,13 @@ int Engine::update(float dt, std::string *err)
     int rc = set_dt(dt, err);
     if (rc) return rc;
     const bool rebuild = rebuild_due();
+    const bool fresh = !recorded_;
+    recorded_ = false;  // until this update is enqueued
     if (prof_active_ && prof_used_ * 6 < prof_ev_.size()) {
-        CK(enqueue_update(&prof_ev_[6 * prof_used_], rebuild), "launch profiled update");
+        CK(enqueue_update(&prof_ev_[6 * prof_used_], rebuild, fresh), "launch profiled update");
         ++prof_used_;
     } else if (timing_) {
-        CK(enqueue_update(ev_, rebuild), "launch update");
+        CK(enqueue_update(ev_, rebuild, fresh), "launch update");
         CK(hipEventSynchronize(ev_[5]), "sync update");
         for (int i = 0; i < 5; ++i) {
             float ms = 0.0f;
@@ -698,10 +706,10 @@ int Engine::update(float dt, std::string *err)
         }
         stage_ms_[5] = 0.0;  // position + collision runs inside the viscosity kernel
     } else if (use_graph_) {
-        const int gi = rebuild ? 1 : 0;
+        const int gi = (rebuild ? 1 : 0) + (fresh ? 2 : 0);
         if (!graph_exec_[gi]) {
             CK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture");
-            hipError_t le = enqueue_update(nullptr, rebuild);
+            hipError_t le = enqueue_update(nullptr, rebuild, fresh);
             hipError_t ce = hipStreamEndCapture(stream_, &graph_[gi]);
             CK(le, "capture update");
             CK(ce, "end capture");
@@ -710,10 +718,11 @@ int Engine::update(float dt, std::string *err)
         }
         CK(hipGraphLaunch(graph_exec_[gi], stream_), "graph launch");
     } else {
-        CK(enqueue_update(nullptr, rebuild), "launch update");
+        CK(enqueue_update(nullptr, rebuild, fresh), "launch update");
     }
     ++work_updates_;
     have_tables_ = true;
+    recorded_ = true;  // for the next update, if nothing changes and dt stays
     return SPH_OK;
 }
 
@@ -742,6 +751,7 @@ int Engine::autotune(float dt, int updates, double ms_out[3], std::string *err)
         return e;
     };
     auto restore = [&]() -> hipError_t {
+        recorded_ = false;  // the state goes back: the next update starts with predict
         hipError_t e = hipMemcpyAsync(d_.pos, snap, sizeof(float4) * n, hipMemcpyDeviceToDevice, stream_);
         if (e == hipSuccess)
             e = hipMemcpyAsync(d_.vel, snap + n, sizeof(float4) * n, hipMemcpyDeviceToDevice, stream_);
@@ -831,6 +841,7 @@ int Engine::autotune(float dt, int updates, double ms_out[3], std::string *err)
     invalidate_graph();
     work_valid_ = false;
     have_tables_ = false;
+    recorded_ = false;  // the trial updates' records belong to a state restored since
     if (ms_out)
         for (int q = 0; q < 3; ++q) ms_out[q] = best[q];
     return rc;
@@ -909,6 +920,7 @@ int Engine::prime(std::string *err)
     // dt = 0 reproduces xp = x + 0 exactly.
     int rc = set_dt(0.0f, err);
     if (rc) return rc;
+    recorded_ = false;
     CK(launch_predict(d_, cfg_, stream_), "prime predict");
     if (!grid_ && cfg_.needs_work()) {
         int wb = 0;

@@ -130,7 +130,7 @@ class Engine {
     int alloc(uint32_t capacity, uint32_t table_capacity, std::string *err);
     int push_descs(std::string *err);
     int set_dt(float dt, std::string *err);
-    hipError_t enqueue_update(hipEvent_t *ev, bool rebuild_work);
+    hipError_t enqueue_update(hipEvent_t *ev, bool rebuild_work, bool fresh);
     // Resident slots of scene s into particle order (ids[k] = p_off + k).
     hipError_t normalize_scene(uint32_t s);
     // Render positions of scene s from its current state (if they exist).
@@ -156,8 +156,13 @@ class Engine {
     bool have_tables_ = false;
     bool grid_ = false;  // cell-grid bucket key (else the reference's hash % table)
     bool use_graph_ = true;
-    hipGraph_t graph_[2] = {nullptr, nullptr};          // [rebuild work order?]
-    hipGraphExec_t graph_exec_[2] = {nullptr, nullptr};
+    // [rebuild work order? + 2 * fresh predict?]
+    hipGraph_t graph_[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipGraphExec_t graph_exec_[4] = {nullptr, nullptr, nullptr, nullptr};
+    // The last update's viscosity pass recorded the next update's buckets
+    // (keys, tickets, counts) and nothing has changed since: state, settings,
+    // layout and dt.  Otherwise the next update starts with predict.
+    bool recorded_ = false;
     uint32_t work_period_ = 8;   // updates between work-order rebuilds
     uint64_t work_updates_ = 0;  // updates since creation
     bool work_valid_ = false;    // a work order has been built

@@ -88,68 +88,102 @@ __device__ __forceinline__ uint32_t work_key(const EngineDesc &E, const SceneDes
 // ---------------------------------------------------------------------------
 // Stage A + B1
 // ---------------------------------------------------------------------------
-// Runs over resident slots g.  count: [table_total] records per bucket (zero
-// on entry), then [n_scenes] flags raised when a particle's cell lies outside
-// the grid's box.  Each record takes a ticket, a distinct place in its bucket
-// in arbitrary order: one atomic per run of equal keys in consecutive lanes
-// (the resident order is nearly sorted, so a wave's 64 records fall in a few
-// runs).  wkeys may be null (no Morton work order).
+// Stage A of one particle (cc:42-48) and its cell (cc:499-503) and hash
+// (cc:505-511).  Every kernel that needs them recomputes them from the same
+// inputs with the same operations, so the values agree to the bit.
+struct Predicted {
+    float vx, vy, vz;  // velocity += CalculateExternalFoce(...) * deltatime (cc:45)
+    float px, py, pz;  // predictedPositions = positions + velocity * (1/120) (cc:47)
+    int cx, cy, cz;
+    uint32_t hash;
+};
+
+__device__ __forceinline__ Predicted predict_particle(const SceneDesc &S, const float4 &x,
+                                                      const float4 &v)
+{
+    Predicted P;
+    const float dt = S.dt;
+    P.vx = v.x + S.g[0] * dt;
+    P.vy = v.y + S.g[1] * dt;
+    P.vz = v.z + S.g[2] * dt;
+    P.px = x.x + P.vx * S.lookahead;
+    P.py = x.y + P.vy * S.lookahead;
+    P.pz = x.z + P.vz * S.lookahead;
+    P.cx = cell_coord(P.px, S.h);
+    P.cy = cell_coord(P.py, S.h);
+    P.cz = cell_coord(P.pz, S.h);
+    P.hash = hash_cell(P.cx, P.cy, P.cz);
+    return P;
+}
+
+// Where the next sort puts a particle: its bucket (GetKeyFromHash, cc:513-516,
+// or the cell-grid key) and a ticket, a distinct place in the bucket in
+// arbitrary order, from count ([table_total] records per bucket, zero before
+// the first record; then [n_scenes] flags raised when a particle's cell lies
+// outside the grid's box).  One atomic per run of equal keys in consecutive
+// lanes: in the resident order, nearly sorted, a wave's 64 records fall in a
+// few runs.  Every lane of the wave that has not exited calls this, the
+// active lanes a prefix of the wave; S is read only when `valid`.
+__device__ __forceinline__ void record_bucket(const EngineDesc &E, const SceneDesc &S,
+                                              const Predicted &P, bool valid, uint32_t g,
+                                              uint32_t *__restrict__ keys,
+                                              uint32_t *__restrict__ ticks,
+                                              uint32_t *__restrict__ count,
+                                              uint32_t *__restrict__ wkeys)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t upto = (2ull << lane) - 1ull;  // lanes 0..lane
+    uint32_t key = 0xffffffffu;
+    if (valid) {
+        key = S.key_off + (S.grid ? grid_key(P.cx, P.cy, P.cz, S.cell_lo, S.gdim, S.gmod)
+                                  : fastmod(P.hash, S.mod_m, S.table));
+        keys[g] = key;
+        // grid key: a cell outside the grid's box wraps onto another cell's bucket
+        if (S.grid && ((uint32_t)P.cx - (uint32_t)S.cell_lo[0] >= S.gdim[0] ||
+                       (uint32_t)P.cy - (uint32_t)S.cell_lo[1] >= S.gdim[1] ||
+                       (uint32_t)P.cz - (uint32_t)S.cell_lo[2] >= S.gdim[2]))
+            count[E.table_total + S.index] = 1u;
+        if (wkeys) wkeys[g] = work_key(E, S, S.index, P.cx, P.cy, P.cz);
+    }
+    // runs of equal keys: a run starts at lane 0, after a different key, and
+    // after a lane without a record (those come after every lane with one)
+    const uint32_t prev = __shfl_up(key, 1u, 64);
+    const uint64_t heads = __ballot(valid && (lane == 0u || prev != key)) | ~__ballot(valid);
+    const uint32_t first = 63u - (uint32_t)__builtin_clzll(heads & upto);  // this run's head
+    const uint64_t later = heads & ~upto;
+    const uint32_t len = (later ? (uint32_t)__builtin_ctzll(later) : 64u) - lane;
+    uint32_t base = 0;
+    if (valid && first == lane) base = atomicAdd(&count[key], len);
+    base = __shfl(base, (int)first, 64);
+    if (valid) ticks[g] = base + (lane - first);
+}
+
+// Stage A + B1 over resident slots g, for an update whose buckets the previous
+// update did not record (the first one, or after a change of state, settings
+// or dt); otherwise the viscosity pass of the previous update recorded them.
 template <bool MULTI>
 __global__ void k_predict(const EngineDesc *__restrict__ ed, const SceneDesc *__restrict__ sd,
                           const uint32_t *__restrict__ scene_of, const float4 *__restrict__ pos,
-                          float4 *__restrict__ vel, float4 *__restrict__ xp,
-                          uint32_t *__restrict__ keys, uint32_t *__restrict__ ticks,
-                          uint32_t *__restrict__ count, uint32_t *__restrict__ wkeys)
+                          const float4 *__restrict__ vel, uint32_t *__restrict__ keys,
+                          uint32_t *__restrict__ ticks, uint32_t *__restrict__ count,
+                          uint32_t *__restrict__ wkeys)
 {
     const EngineDesc E = *ed;
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint64_t upto = (2ull << lane) - 1ull;  // lanes 0..lane
-    // whole waves iterate together (the ballots below need every lane)
+    // whole waves iterate together (record_bucket's ballots need every lane)
     for (uint32_t g0 = blockIdx.x * blockDim.x; g0 < E.n_total; g0 += grid_stride()) {
         const uint32_t g = g0 + threadIdx.x;
         const bool valid = g < E.n_total;
-        uint32_t key = 0xffffffffu;
-        if (valid) {
-            const uint32_t s = MULTI ? scene_of[g] : 0u;
-            const SceneDesc &S = sd[s];
-            const float dt = S.dt;
-            float4 v = vel[g];
-            const float4 x = pos[g];
-            // cc:45: velocity += CalculateExternalFoce(...) * deltatime
-            v.x = v.x + S.g[0] * dt;
-            v.y = v.y + S.g[1] * dt;
-            v.z = v.z + S.g[2] * dt;
-            // cc:47: predictedPositions = positions + velocity * (1/120)
-            const float px = x.x + v.x * S.lookahead;
-            const float py = x.y + v.y * S.lookahead;
-            const float pz = x.z + v.z * S.lookahead;
-            const int cx = cell_coord(px, S.h), cy = cell_coord(py, S.h), cz = cell_coord(pz, S.h);
-            const uint32_t hash = hash_cell(cx, cy, cz);
-            vel[g] = v;
-            xp[g] = make_float4(px, py, pz, __uint_as_float(hash));
-            // bucket: GetKeyFromHash (cc:513-516), or the cell-grid key
-            key = S.key_off + (S.grid ? grid_key(cx, cy, cz, S.cell_lo, S.gdim, S.gmod)
-                                      : fastmod(hash, S.mod_m, S.table));
-            keys[g] = key;
-            // grid key: a cell outside the grid's box wraps onto another cell's bucket
-            if (S.grid && ((uint32_t)cx - (uint32_t)S.cell_lo[0] >= S.gdim[0] ||
-                           (uint32_t)cy - (uint32_t)S.cell_lo[1] >= S.gdim[1] ||
-                           (uint32_t)cz - (uint32_t)S.cell_lo[2] >= S.gdim[2]))
-                count[E.table_total + S.index] = 1u;
-            if (wkeys) wkeys[g] = work_key(E, S, s, cx, cy, cz);
-        }
-        // runs of equal keys: a run starts at lane 0, after a different key,
-        // and after an invalid lane (only the last wave has any, at its end)
-        const uint32_t prev = __shfl_up(key, 1u, 64);
-        const uint64_t heads = __ballot(valid && (lane == 0u || prev != key)) | ~__ballot(valid);
-        const uint32_t first = 63u - (uint32_t)__builtin_clzll(heads & upto);  // this run's head
-        const uint64_t later = heads & ~upto;
-        const uint32_t len = (later ? (uint32_t)__builtin_ctzll(later) : 64u) - lane;
-        uint32_t base = 0;
-        if (valid && first == lane) base = atomicAdd(&count[key], len);
-        base = __shfl(base, (int)first, 64);
-        if (valid) ticks[g] = base + (lane - first);
+        const SceneDesc &S = sd[MULTI && valid ? scene_of[g] : 0u];
+        Predicted P{};
+        if (valid) P = predict_particle(S, pos[g], vel[g]);
+        record_bucket(E, S, P, valid, g, keys, ticks, count, wkeys);
     }
+}
+
+// count[0, n) <- 0: the counts and flags of a recorded update that will not run.
+__global__ void k_count_clear(uint32_t *__restrict__ count, uint32_t n)
+{
+    for (uint32_t i = global_id(); i < n; i += grid_stride()) count[i] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -425,17 +459,22 @@ __global__ void k_count_scatter(const EngineDesc *__restrict__ ed,
 }
 
 // A record's sorted slot is its bucket's first slot plus the number of the
-// bucket's records with a smaller particle index.  Writes the sorted copies
-// (xps with w = float(hash), the reference record's .y (cc:480); vs; ps with
-// w = the exact hash bits) and sids, the particle index of each sorted slot.
+// bucket's records with a smaller particle index.  Recomputes stage A from the
+// resident state and writes the sorted copies: xps (predicted position, w =
+// float(hash), the reference record's .y, cc:480), vs (velocity after the
+// external force), ps (position, w = the exact hash bits) and sids, the
+// particle index of each sorted slot.
+template <bool MULTI>
 __global__ void k_count_reorder(const EngineDesc *__restrict__ ed,
+                                const SceneDesc *__restrict__ sd,
+                                const uint32_t *__restrict__ scene_of,
                                 const uint32_t *__restrict__ keys,
                                 const uint32_t *__restrict__ bounds,
                                 const uint2 *__restrict__ order, uint32_t *__restrict__ sids,
-                                const float4 *__restrict__ xp, const float4 *__restrict__ vel,
-                                const float4 *__restrict__ pos, float4 *__restrict__ xps,
-                                float4 *__restrict__ vs, float4 *__restrict__ ps,
-                                uint32_t *__restrict__ rank, uint32_t *__restrict__ count)
+                                const float4 *__restrict__ vel, const float4 *__restrict__ pos,
+                                float4 *__restrict__ xps, float4 *__restrict__ vs,
+                                float4 *__restrict__ ps, uint32_t *__restrict__ rank,
+                                uint32_t *__restrict__ count)
 {
     const uint32_t n = ed->n_total;
     for (uint32_t t = global_id(); t < n; t += grid_stride()) {
@@ -447,11 +486,13 @@ __global__ void k_count_reorder(const EngineDesc *__restrict__ ed,
         for (uint32_t j = lo; j < hi; ++j) before += order[j].y < o.y ? 1u : 0u;
         const uint32_t slot = lo + before;
         if (slot >= hi) continue;  // as above
-        const float4 r = xp[o.x], x0 = pos[o.x];
+        const SceneDesc &S = scene_of_slot<MULTI>(sd, scene_of, o.x);
+        const float4 x0 = pos[o.x], v0 = vel[o.x];
+        const Predicted P = predict_particle(S, x0, v0);
         sids[slot] = o.y;
-        xps[slot] = make_float4(r.x, r.y, r.z, (float)__float_as_uint(r.w));
-        vs[slot] = vel[o.x];
-        ps[slot] = make_float4(x0.x, x0.y, x0.z, r.w);
+        xps[slot] = make_float4(P.px, P.py, P.pz, (float)P.hash);
+        vs[slot] = make_float4(P.vx, P.vy, P.vz, v0.w);
+        ps[slot] = make_float4(x0.x, x0.y, x0.z, __uint_as_float(P.hash));
         if (rank) rank[o.y] = slot;  // only the Morton work order reads it
     }
     const uint32_t nc = ed->table_total + ed->n_scenes;
@@ -1202,8 +1243,9 @@ __global__ __launch_bounds__(256) void k_pressure(SPH_GATHER_PARAMS, const float
 }
 
 // Writes the new state at the sorted slot (pos, vel, ids: the next resident
-// order); rpos and colors (may be null) are the render buffers, in particle
-// order.
+// order) and records each particle's bucket for the next update's sort, with
+// this update's dt (record_bucket: keys, ticks, count, wkeys); rpos and
+// colors (may be null) are the render buffers, in particle order.
 template <bool MULTI, bool WORK, bool LIST, bool NBR, bool GRID>
 __global__ __launch_bounds__(256, MULTI ? 4 : 5) void k_viscosity_integrate(SPH_GATHER_PARAMS,
                                                               const float4 *__restrict__ vs2,
@@ -1212,7 +1254,11 @@ __global__ __launch_bounds__(256, MULTI ? 4 : 5) void k_viscosity_integrate(SPH_
                                                               float4 *__restrict__ vel,
                                                               uint32_t *__restrict__ ids,
                                                               float4 *__restrict__ rpos,
-                                                              float4 *__restrict__ colors)
+                                                              float4 *__restrict__ colors,
+                                                              uint32_t *__restrict__ keys,
+                                                              uint32_t *__restrict__ ticks,
+                                                              uint32_t *__restrict__ count,
+                                                              uint32_t *__restrict__ wkeys)
 {
     constexpr bool CONSUMER = NBR && GRID;  // reads the neighbour masks
     SPH_GATHER_PROLOGUE();
@@ -1260,6 +1306,8 @@ __global__ __launch_bounds__(256, MULTI ? 4 : 5) void k_viscosity_integrate(SPH_
         speed_color(v.x, v.y, v.z, ed->stops, c);
         colors[g] = make_float4(c[0], c[1], c[2], c[3]);
     }
+    // stage A + B1 of the next update, from the new state
+    record_bucket(*ed, S, predict_particle(S, x, v), true, k, keys, ticks, count, wkeys);
 }
 
 // ---------------------------------------------------------------------------
@@ -1408,11 +1456,13 @@ hipError_t launch_predict(const DevState &s, const LaunchCfg &c, hipStream_t st)
 {
     const uint32_t g = grid_for(s.capacity, c.block_stream);
     uint32_t *const wk = s.grid ? nullptr : s.wkeys[0];  // no work order under the grid key
+    k_count_clear<<<grid_for(s.count_size, c.block_stream), c.block_stream, 0, st>>>(s.count,
+                                                                                    s.count_size);
     if (s.multi)
-        k_predict<true><<<g, c.block_stream, 0, st>>>(s.ed, s.sd, s.scene_of, s.pos, s.vel, s.xp,
+        k_predict<true><<<g, c.block_stream, 0, st>>>(s.ed, s.sd, s.scene_of, s.pos, s.vel,
                                                        s.keys[0], s.keys[1], s.count, wk);
     else
-        k_predict<false><<<g, c.block_stream, 0, st>>>(s.ed, s.sd, s.scene_of, s.pos, s.vel, s.xp,
+        k_predict<false><<<g, c.block_stream, 0, st>>>(s.ed, s.sd, s.scene_of, s.pos, s.vel,
                                                         s.keys[0], s.keys[1], s.count, wk);
     SPH_LAUNCH_CHECK();
 }
@@ -1444,9 +1494,17 @@ hipError_t launch_bucket_sort(const DevState &s, const LaunchCfg &c, hipStream_t
     k_count_scan<<<nb, kSortBlock, 0, st>>>(s.ed, s.count, s.parts, s.bounds);
     k_count_scatter<<<g, c.block_stream, 0, st>>>(s.ed, s.keys[0], s.keys[1], s.ids, s.bounds,
                                                    s.order);
-    k_count_reorder<<<g, c.block_stream, 0, st>>>(s.ed, s.keys[0], s.bounds, s.order, s.sids,
-                                                   s.xp, s.vel, s.pos, s.xps, s.vs, s.ps,
-                                                   s.grid ? nullptr : s.rank, s.count);
+    uint32_t *const rk = s.grid ? nullptr : s.rank;
+    if (s.multi)
+        k_count_reorder<true><<<g, c.block_stream, 0, st>>>(s.ed, s.sd, s.scene_of, s.keys[0],
+                                                             s.bounds, s.order, s.sids, s.vel,
+                                                             s.pos, s.xps, s.vs, s.ps, rk,
+                                                             s.count);
+    else
+        k_count_reorder<false><<<g, c.block_stream, 0, st>>>(s.ed, s.sd, s.scene_of, s.keys[0],
+                                                              s.bounds, s.order, s.sids, s.vel,
+                                                              s.pos, s.xps, s.vs, s.ps, rk,
+                                                              s.count);
     SPH_LAUNCH_CHECK();
 }
 
@@ -1535,7 +1593,8 @@ hipError_t launch_pressure(const DevState &s, const LaunchCfg &c, hipStream_t st
 hipError_t launch_viscosity_integrate(const DevState &s, const LaunchCfg &c, hipStream_t st)
 {
     SPH_GATHER_LAUNCH(k_viscosity_integrate, c.viscosity, c.viscosity.block, st, SPH_GATHER_ARGS,
-                      s.vs2, s.ps, s.pos, s.vel, s.ids, s.rpos, s.colors);
+                      s.vs2, s.ps, s.pos, s.vel, s.ids, s.rpos, s.colors, s.keys[0], s.keys[1],
+                      s.count, s.grid ? nullptr : s.wkeys[0]);
     SPH_LAUNCH_CHECK();
 }
 

@@ -22,14 +22,15 @@ struct DevState {
     float4 *pos;    // x,y,z, w = 0.34 after an update (OutPositions layout, cc:107)
     float4 *vel;    // x,y,z, w unused
     uint32_t *ids;  // particle index (engine-wide) of each resident slot
-    float4 *xp;     // predicted position, w = bits of the exact u32 hash (resident order)
-    // bucket sort: keys[0] bucket key and keys[1] ticket by resident slot;
+    // bucket sort: keys[0] bucket key and keys[1] ticket by resident slot
+    // (recorded by the previous update's viscosity pass, or by predict);
     // order the (resident slot, particle index) pairs of each bucket in
     // ticket order
     uint32_t *keys[2];
     uint2 *order;
-    uint32_t *count;   // [table_capacity + scene capacity] bucket counts, then scene flags;
-                       // all zero between updates
+    uint32_t *count;   // [count_size = table_capacity + scene capacity] bucket counts, then
+                       // scene flags: the next update's records, or zero
+    uint32_t count_size;
     uint32_t *parts;   // [count_blocks + 1] chunk sums, then the last-block counter
     uint32_t count_blocks;
     // radix sort of the Morton work order
@@ -101,7 +102,8 @@ inline LaunchCfg default_launch_cfg(bool grid)
     return sanitize_launch_cfg(c, grid);
 }
 
-// Stage A + B1: predict, cell, hash, bucket key; counts each bucket's records.
+// Stage A + B1 when the previous update did not record the buckets: clears
+// the counts, then predict, cell, hash, bucket key and ticket of every particle.
 hipError_t launch_predict(const DevState &s, const LaunchCfg &c, hipStream_t st);
 // Stages B2 + B3: counting sort into (bucket, particle index) order, bucket
 // bounds, gather of xp, v into sorted order; clears the counts.

@@ -125,3 +125,29 @@ def test_render_buffers_hold_particle_order():
         # an upload rewrites the render positions at once
         sim.upload(sc.pos, sc.vel)
         assert_bitexact(_read_f4(pos_ptr, sc.n), sc.pos, "render positions after upload")
+
+
+def test_changing_dt_and_settings_between_updates():
+    """Each update's viscosity pass records the next update's buckets with its
+    own dt; a different dt, new settings or a new state must not reuse them."""
+    sc = _scene()
+    announce("dt and settings changes", sc.name)
+    dts = [DT, DT, 0.5 * DT, 0.5 * DT, DT, 0.0, DT]
+    p2 = S.default_params(gravity_on=1, gravity_scale=3.0, viscosity=0.5, bounds=(4.0, 4.0, 4.0))
+    with S.Sim(sc.params, sc.n) as sim:
+        sim.upload(sc.pos, sc.vel)
+        pos, vel = sc.pos, sc.vel
+        for dt in dts:
+            sim.update(dt)
+            pos, vel, dens = O.update(orc_params(sc.params), dt, pos, vel, T=sc.n)
+        pg, vg, dg = sim.download()
+        assert_bitexact(pg[:, :3], pos[:, :3], "positions over changing dt")
+        assert_bitexact(vg[:, :3], vel[:, :3], "velocities over changing dt")
+        assert_bitexact(dg, dens, "densities over changing dt")
+        sim.params = p2  # new settings between two updates with the same dt
+        for _ in range(2):
+            sim.update(DT)
+            pos, vel, dens = O.update(orc_params(p2), DT, pos, vel, T=sc.n)
+        pg, vg, dg = sim.download()
+        assert_bitexact(pg[:, :3], pos[:, :3], "positions after a settings change")
+        assert_bitexact(dg, dens, "densities after a settings change")
