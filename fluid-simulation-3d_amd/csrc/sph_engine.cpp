@@ -326,8 +326,8 @@ int Engine::alloc(uint32_t cap, uint32_t tcap, std::string *err)
     const size_t count_bytes = sizeof(uint32_t) * (size_t)d_.count_size;
     if (!rc) rc = dm((void **)&d_.count, count_bytes, "bucket counts");
     if (!rc) rc = check(hipMemset(d_.count, 0, count_bytes), "bucket counts", err);
-    if (!rc) rc = dm((void **)&d_.parts, sizeof(uint32_t) * ((size_t)cblk + 2), "count chunks");
-    if (!rc) rc = check(hipMemset(d_.parts, 0, sizeof(uint32_t) * ((size_t)cblk + 2)), "count chunks", err);
+    if (!rc) rc = dm((void **)&d_.chain, sizeof(uint64_t) * ((size_t)cblk + 1), "scan chain");
+    if (!rc) rc = check(hipMemset(d_.chain, 0, sizeof(uint64_t) * ((size_t)cblk + 1)), "scan chain", err);
     for (int b = 0; b < 2 && !rc; ++b) {
         rc = dm((void **)&d_.wkeys[b], u4, "work keys");
         if (!rc) rc = dm((void **)&d_.wvals[b], u4, "work order");
@@ -352,7 +352,7 @@ Engine::~Engine()
     void *bufs[] = {d_ed_,       d_sd_,       d_scene_of_, d_.pos,     d_.vel,
                     d_.xps,      d_.vs,       d_.vs2,      d_.dens,    d_.keys[0], d_.keys[1],
                     d_.ids,      d_.ps,       d_.order,    d_.sids,
-                    d_.hist,     d_.bounds,   d_.count,    d_.parts,
+                    d_.hist,     d_.bounds,   d_.count,    d_.chain,
                     d_.wkeys[0], d_.wkeys[1], d_.wvals[0], d_.wvals[1], d_.rank,
                     d_.nbr,      d_.ncnt,     d_.sflags};
     for (void *b : bufs)

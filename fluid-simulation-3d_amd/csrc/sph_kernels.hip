@@ -6,7 +6,7 @@
 // Pipeline of one Update(dt) over all scenes of an engine:
 //   k_predict          stage A + B1  (cc:42-48, 473-482) in resident order;
 //                      counts the records of each bucket
-//   k_count_reduce/scan/scatter/reorder  stages B2 + B3 (cc:481-496): a
+//   k_count_scan/scatter/reorder  stages B2 + B3 (cc:481-496): a
 //                      counting sort over the buckets, ordered within a bucket
 //                      by particle index (the stable (key, index) order);
 //                      bucket bounds; gather of the state into sorted order
@@ -336,10 +336,11 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_scatter(
 // particle index within a bucket, and the table is the exclusive prefix sum of
 // the bucket counts: bucket b holds slots [bounds[b], bounds[b+1]), empty when
 // they are equal (the reference's INT_MAX start, cc:481).
-//   k_predict        count[key]++ and a ticket per particle (above)
-//   k_count_reduce   per-chunk sums; the last block to finish scans them and
-//                    publishes the scene flags
-//   k_count_scan     bounds from the chunk bases and the counts
+//   k_predict        count[key]++ and a ticket per particle (above), or the
+//                    previous update's viscosity pass
+//   k_count_scan     bounds: one pass over the counts in chunks, with the
+//                    chunk prefixes by decoupled look-back; publishes the
+//                    scene flags
 //   k_count_scatter  tmp[bounds[key] + ticket] = slot: each bucket's slots, in
 //                    ticket order
 //   k_count_reorder  a record's place in its bucket is the number of the
@@ -363,74 +364,87 @@ __device__ __forceinline__ void load_counts(const uint32_t *__restrict__ count, 
     }
 }
 
-// parts: [gridDim.x + 1]; done: a counter that is zero on entry and on exit.
-__global__ __launch_bounds__(kSortBlock) void k_count_reduce(const EngineDesc *__restrict__ ed,
-                                                              const uint32_t *__restrict__ count,
-                                                              uint32_t *__restrict__ parts,
-                                                              uint32_t *__restrict__ done,
-                                                              uint32_t *__restrict__ sflags)
+// Bucket bounds in one pass (decoupled look-back).  Each workgroup takes the
+// next chunk in launch order from a counter, publishes the chunk's count sum,
+// then adds the sums of the chunks before it -- its first wave reads 64 status
+// words at a time, back to the nearest chunk whose inclusive prefix is
+// published -- and publishes its own inclusive prefix.  A workgroup waits only
+// for chunks taken before its own, by workgroups already running, so every
+// wait ends; the wait is bounded as well (past the bound the result is wrong,
+// never a hang).  chain: [gridDim.x] status words, then the chunk counter; all
+// zero on entry (the reorder clears them).  The workgroup of the last chunk
+// writes bounds[T] and publishes the scene flags.
+constexpr uint64_t kChainAgg = 1ull << 62;    // the chunk's count sum
+constexpr uint64_t kChainInc = 2ull << 62;    // the inclusive prefix through the chunk
+constexpr uint64_t kChainFlags = 3ull << 62;
+constexpr uint32_t kChainSpinMax = 1u << 16;  // sleeps of 64 clocks
+
+__device__ __forceinline__ uint64_t chain_load(const uint64_t *p)
 {
-    __shared__ uint32_t wsum[kSortWaves];
-    __shared__ uint32_t last;
-    const uint32_t T = ed->table_total, tid = threadIdx.x;
-    uint32_t c[kCountPer];
-    load_counts(count, blockIdx.x * kCountChunk + tid * kCountPer, T, c);
-    uint32_t sum = 0;
-#pragma unroll
-    for (uint32_t j = 0; j < kCountPer; ++j) sum += c[j];
-    uint32_t total;
-    (void)block_scan_excl(sum, wsum, &total);
-    if (tid == 0) {
-        parts[blockIdx.x] = total;
-        __threadfence();  // the sum is visible before the counter moves
-        last = atomicAdd(done, 1u) == gridDim.x - 1u;
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();  // every other block's sum is visible here
-    // the last block: exclusive scan of the chunk sums, in place; each
-    // work-item scans a contiguous run of them
-    const uint32_t nb = gridDim.x;
-    const uint32_t per = (nb + kSortBlock - 1) / kSortBlock;
-    const uint32_t lo = min(nb, tid * per), hi = min(nb, lo + per);
-    uint32_t run = 0;
-    for (uint32_t i = lo; i < hi; ++i) run += parts[i];
-    uint32_t all;
-    uint32_t pre = block_scan_excl(run, wsum, &all);
-    for (uint32_t i = lo; i < hi; ++i) {
-        const uint32_t v = parts[i];
-        parts[i] = pre;
-        pre += v;
-    }
-    if (tid == 0) {
-        parts[nb] = all;
-        *done = 0u;
-    }
-    for (uint32_t q = tid; q < ed->n_scenes; q += kSortBlock) sflags[q] = count[T + q];
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void chain_store(uint64_t *p, uint64_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __global__ __launch_bounds__(kSortBlock) void k_count_scan(const EngineDesc *__restrict__ ed,
                                                             const uint32_t *__restrict__ count,
-                                                            const uint32_t *__restrict__ parts,
-                                                            uint32_t *__restrict__ bounds)
+                                                            uint64_t *__restrict__ chain,
+                                                            uint32_t *__restrict__ bounds,
+                                                            uint32_t *__restrict__ sflags)
 {
     __shared__ uint32_t wsum[kSortWaves];
-    const uint32_t T = ed->table_total, tid = threadIdx.x;
-    const uint32_t b0 = blockIdx.x * kCountChunk + tid * kCountPer;
-    uint32_t c[kCountPer];
-    load_counts(count, b0, T, c);
+    __shared__ uint32_t chunk_s, base_s;
+    const uint32_t T = ed->table_total, tid = threadIdx.x, nb = gridDim.x;
+    if (tid == 0) chunk_s = atomicAdd(reinterpret_cast<uint32_t *>(chain + nb), 1u);
+    __syncthreads();
+    const uint32_t c = chunk_s;
+    if (c >= nb) return;  // never taken while the counter starts at zero
+    const uint32_t b0 = c * kCountChunk + tid * kCountPer;
+    uint32_t cnt[kCountPer];
+    load_counts(count, b0, T, cnt);
     uint32_t sum = 0;
 #pragma unroll
-    for (uint32_t j = 0; j < kCountPer; ++j) sum += c[j];
+    for (uint32_t j = 0; j < kCountPer; ++j) sum += cnt[j];
     uint32_t total;
-    uint32_t p = parts[blockIdx.x] + block_scan_excl(sum, wsum, &total);
+    const uint32_t local = block_scan_excl(sum, wsum, &total);
+    if (tid < 64) {
+        const uint32_t lane = tid;
+        if (lane == 0) chain_store(&chain[c], (c == 0 ? kChainInc : kChainAgg) | total);
+        uint32_t prefix = 0, spins = 0;
+        for (int32_t j0 = (int32_t)c - 1; j0 >= 0; j0 -= 64) {
+            const int32_t j = j0 - (int32_t)lane;  // lane 0: the nearest chunk
+            uint64_t st = j >= 0 ? chain_load(&chain[j]) : kChainInc;  // before chunk 0: 0
+            while (__ballot((st & kChainFlags) == 0) != 0 && spins < kChainSpinMax) {
+                __builtin_amdgcn_s_sleep(1);
+                ++spins;
+                if ((st & kChainFlags) == 0) st = chain_load(&chain[j]);
+            }
+            const uint64_t inc = __ballot((st & kChainFlags) == kChainInc);
+            // the chunks up to the nearest inclusive prefix (all 64 if none)
+            const uint32_t stop = inc ? (uint32_t)__builtin_ctzll(inc) : 63u;
+            uint32_t v = lane <= stop ? (uint32_t)st : 0u;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            prefix += v;
+            if (inc) break;
+        }
+        if (lane == 0) {
+            if (c > 0) chain_store(&chain[c], kChainInc | (uint64_t)(prefix + total));
+            base_s = prefix;
+        }
+    }
+    __syncthreads();
+    uint32_t p = base_s + local;
     if (b0 + kCountPer <= T) {
         uint4 *q = reinterpret_cast<uint4 *>(bounds + b0);
         uint32_t o[kCountPer];
 #pragma unroll
         for (uint32_t j = 0; j < kCountPer; ++j) {
             o[j] = p;
-            p += c[j];
+            p += cnt[j];
         }
         q[0] = make_uint4(o[0], o[1], o[2], o[3]);
         q[1] = make_uint4(o[4], o[5], o[6], o[7]);
@@ -438,10 +452,13 @@ __global__ __launch_bounds__(kSortBlock) void k_count_scan(const EngineDesc *__r
 #pragma unroll
         for (uint32_t j = 0; j < kCountPer; ++j) {
             if (b0 + j < T) bounds[b0 + j] = p;
-            p += c[j];
+            p += cnt[j];
         }
     }
-    if (blockIdx.x == 0 && tid == 0) bounds[T] = parts[gridDim.x];
+    if (c == nb - 1) {
+        if (tid == 0) bounds[T] = base_s + total;
+        for (uint32_t q = tid; q < ed->n_scenes; q += kSortBlock) sflags[q] = count[T + q];
+    }
 }
 
 // order[bounds[key] + ticket] = (resident slot, particle index)
@@ -474,7 +491,8 @@ __global__ void k_count_reorder(const EngineDesc *__restrict__ ed,
                                 const float4 *__restrict__ vel, const float4 *__restrict__ pos,
                                 float4 *__restrict__ xps, float4 *__restrict__ vs,
                                 float4 *__restrict__ ps, uint32_t *__restrict__ rank,
-                                uint32_t *__restrict__ count)
+                                uint32_t *__restrict__ count, uint64_t *__restrict__ chain,
+                                uint32_t chain_words)
 {
     const uint32_t n = ed->n_total;
     for (uint32_t t = global_id(); t < n; t += grid_stride()) {
@@ -497,6 +515,7 @@ __global__ void k_count_reorder(const EngineDesc *__restrict__ ed,
     }
     const uint32_t nc = ed->table_total + ed->n_scenes;
     for (uint32_t i = global_id(); i < nc; i += grid_stride()) count[i] = 0u;
+    for (uint32_t i = global_id(); i < chain_words; i += grid_stride()) chain[i] = 0u;
 }
 
 // Resident slots [0, n) of one scene into particle order:
@@ -1490,8 +1509,7 @@ hipError_t launch_radix_sort(const DevState &s, uint32_t *const keys[2], uint32_
 hipError_t launch_bucket_sort(const DevState &s, const LaunchCfg &c, hipStream_t st)
 {
     const uint32_t nb = s.count_blocks, g = grid_for(s.capacity, c.block_stream);
-    k_count_reduce<<<nb, kSortBlock, 0, st>>>(s.ed, s.count, s.parts, s.parts + nb + 1, s.sflags);
-    k_count_scan<<<nb, kSortBlock, 0, st>>>(s.ed, s.count, s.parts, s.bounds);
+    k_count_scan<<<nb, kSortBlock, 0, st>>>(s.ed, s.count, s.chain, s.bounds, s.sflags);
     k_count_scatter<<<g, c.block_stream, 0, st>>>(s.ed, s.keys[0], s.keys[1], s.ids, s.bounds,
                                                    s.order);
     uint32_t *const rk = s.grid ? nullptr : s.rank;
@@ -1499,12 +1517,12 @@ hipError_t launch_bucket_sort(const DevState &s, const LaunchCfg &c, hipStream_t
         k_count_reorder<true><<<g, c.block_stream, 0, st>>>(s.ed, s.sd, s.scene_of, s.keys[0],
                                                              s.bounds, s.order, s.sids, s.vel,
                                                              s.pos, s.xps, s.vs, s.ps, rk,
-                                                             s.count);
+                                                             s.count, s.chain, nb + 1);
     else
         k_count_reorder<false><<<g, c.block_stream, 0, st>>>(s.ed, s.sd, s.scene_of, s.keys[0],
                                                               s.bounds, s.order, s.sids, s.vel,
                                                               s.pos, s.xps, s.vs, s.ps, rk,
-                                                              s.count);
+                                                              s.count, s.chain, nb + 1);
     SPH_LAUNCH_CHECK();
 }
 

@@ -31,7 +31,8 @@ struct DevState {
     uint32_t *count;   // [count_size = table_capacity + scene capacity] bucket counts, then
                        // scene flags: the next update's records, or zero
     uint32_t count_size;
-    uint32_t *parts;   // [count_blocks + 1] chunk sums, then the last-block counter
+    uint64_t *chain;   // [count_blocks + 1] chunk status words of the one-pass scan, then
+                       // its chunk counter; zero between updates
     uint32_t count_blocks;
     // radix sort of the Morton work order
     uint32_t *hist;        // [radix * radix_blocks] counts, then [radix] digit totals
