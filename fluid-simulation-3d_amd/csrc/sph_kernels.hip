@@ -4,15 +4,19 @@
 // (FluidSimulation::Update, cc:39-111) and engine/physics/kernels.h.
 //
 // Pipeline of one Update(dt) over all scenes of an engine:
-//   k_predict          stage A + B1  (cc:42-48, 473-482) in resident order;
-//                      counts the records of each bucket
+//   (k_count_clear, k_predict)  stage A + B1 (cc:42-48, 473-482) in resident
+//                      order: each particle's bucket and a ticket in it.  Only
+//                      when the previous update did not record them already
 //   k_count_scan/scatter/reorder  stages B2 + B3 (cc:481-496): a
 //                      counting sort over the buckets, ordered within a bucket
 //                      by particle index (the stable (key, index) order);
-//                      bucket bounds; gather of the state into sorted order
+//                      bucket bounds; stage A recomputed into sorted order
 //   k_density          stage C (cc:304-311, 325-365)
 //   k_pressure         stage D (cc:62-69, 367-422)
-//   k_viscosity_integrate  stage E (cc:71-78, 424-464, Jacobi) + F (cc:80-108)
+//   k_viscosity_integrate  stage E (cc:71-78, 424-464, Jacobi) + F (cc:80-108),
+//                      the new state written in sorted order (the next
+//                      resident order), and stage A + B1 of the next update
+//                      recorded from it
 //
 // Every floating-point expression keeps the reference's operation order and
 // rounding (the library is compiled with -ffp-contract=off and IEEE division

@@ -33,7 +33,7 @@
 struct SceneDesc {
     uint32_t p_off;       // first particle slot
     uint32_t n;           // particle count
-    uint32_t key_off;     // first bucket in the global start/end tables
+    uint32_t key_off;     // first bucket in the engine-wide bucket tables
     uint32_t table;       // bucket count T (reference: numParticles, cc:479)
     uint64_t mod_m;       // Lemire remainder constant: floor((2^64-1)/table)+1
     float h;              // interactionRadius
