@@ -916,21 +916,33 @@ int Engine::sync(std::string *err)
 int Engine::prime(std::string *err)
 {
     // InitializeData (cc:144-145) runs UpdateSpatialLookup + updateDensities on
-    // predictedPositions = positions (cc:550) with zero velocity: stage A with
-    // dt = 0 reproduces xp = x + 0 exactly.
+    // predictedPositions = positions (cc:550): stage A with dt = 0 and no
+    // lookahead gives xp = x exactly, whatever the velocities.
     int rc = set_dt(0.0f, err);
     if (rc) return rc;
     recorded_ = false;
-    CK(launch_predict(d_, cfg_, stream_), "prime predict");
-    if (!grid_ && cfg_.needs_work()) {
-        int wb = 0;
-        CK(launch_radix_sort(d_, d_.wkeys, d_.wvals, d_.ids, 32, stream_, &wb), "prime work order");
-        d_.work = d_.wvals[wb];
-        work_valid_ = true;
+    std::vector<float> lookahead(sd_h_.size());
+    for (size_t q = 0; q < sd_h_.size(); ++q) {
+        lookahead[q] = sd_h_[q].lookahead;
+        sd_h_[q].lookahead = 0.0f;
     }
-    CK(launch_bucket_sort(d_, cfg_, stream_), "prime sort");
-    CK(launch_density(d_, cfg_, stream_), "prime density");
-    CK(hipStreamSynchronize(stream_), "prime sync");
+    rc = push_descs(err);
+    hipError_t e = rc ? hipSuccess : launch_predict(d_, cfg_, stream_);
+    if (!rc && e == hipSuccess && !grid_ && cfg_.needs_work()) {
+        int wb = 0;
+        e = launch_radix_sort(d_, d_.wkeys, d_.wvals, d_.ids, 32, stream_, &wb);
+        d_.work = d_.wvals[wb];
+        work_valid_ = e == hipSuccess;
+    }
+    if (!rc && e == hipSuccess) e = launch_bucket_sort(d_, cfg_, stream_);
+    if (!rc && e == hipSuccess) e = launch_density(d_, cfg_, stream_);
+    if (!rc && e == hipSuccess) e = hipStreamSynchronize(stream_);
+    // the settings' lookahead again for the updates
+    for (size_t q = 0; q < sd_h_.size(); ++q) sd_h_[q].lookahead = lookahead[q];
+    const int rc2 = push_descs(err);
+    if (rc) return rc;
+    CK(e, "prime");
+    if (rc2) return rc2;
     have_tables_ = true;
     return SPH_OK;
 }

@@ -31,6 +31,9 @@ KINDS = [
     (S.INIT_UNIFORM_LOWER, 7000, (6.0, 3.0, 6.0), 0.0, 0.0, 0.0),
     (S.INIT_TWO_BLOCKS, 8001, (8.0, 4.0, 8.0), 0.1, 0.002, 2.0),
     (S.INIT_CENTRED_BLOCK, 4913, (4.0, 4.0, 4.0), 0.1, 0.003, 0.0),
+    # moving blocks that straddle |x| = 2: the densities of predictedPositions =
+    # positions (cc:550) differ from those of positions + velocity / 120
+    (S.INIT_TWO_BLOCKS, 6000, (4.4, 4.0, 4.4), 0.1, 0.002, 1.7),
 ]
 
 
