@@ -31,14 +31,14 @@ from sph_amd import workloads as W  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 # Algorithmic HBM bytes per particle per launch of each timed stage: each
 # array read or written once, neighbour re-reads counted as on-chip.
-#   predict:   R pos, vel (16+16), W vel, xp (16+16), W key, ticket (4+4)
-#   density:   R xp (16), W rho, rho_n, 1/rho, 1/rho_n (16); bucket table reads ~0
-#   pressure:  R xp, dens, v (48), W v (16)
-#   viscosity+integrate: R xp, v, pos, id (52), W pos, vel, id (36), and the
+#   predict:   R pos, vel (16+16), W key, ticket (4+4)
+#   density:   R xps (16), W rho, rho_n, 1/rho, 1/rho_n (16); bucket table reads ~0
+#   pressure:  R xps, dens, v (48), W v (16)
+#   viscosity+integrate: R xps, v, pos, id (52), W pos, vel, id (36), and the
 #     next update's bucket key and ticket (8)
 #   (predict runs on its own only when the previous update did not record
 #   the buckets: first update, changed state, settings or dt)
-ALG_BYTES = {"predict": 72.0, "density": 32.0, "pressure": 64.0, "viscosity_integrate": 96.0}
+ALG_BYTES = {"predict": 40.0, "density": 32.0, "pressure": 64.0, "viscosity_integrate": 96.0}
 STAGES = ["predict", "spatial", "density", "pressure", "viscosity_integrate"]
 # Per-dispatch HBM traffic of each kernel instantiation on uniform_1m, from two
 # rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of this script, each scaled by

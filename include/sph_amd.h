@@ -169,8 +169,10 @@ int sph_stage_ms(sph_handle h, double ms[6]);
 /* Per-stage device timing of the next `max_updates` updates (up to 2^20)
  * without synchronising: events bracket each stage on the handle's stream.
  * sph_profile_end waits, returns the mean ms per update of each stage (order
- * of sph_stage_ms; the kernels are: 0 predict+hash, 1 table clear + radix sort
- * + reorder, 2 density, 3 pressure, 4 viscosity+integrate) and the number of
+ * of sph_stage_ms; the kernels are: 0 predict+hash (only when the previous
+ * update did not record this one's buckets, else empty), 1 count scan +
+ * scatter + reorder, 2 density, 3 pressure, 4 viscosity+integrate, which
+ * also records the next update's buckets) and the number of
  * updates profiled.  No reference counterpart: measurement support for the
  * stage timers of physicsWorld.h:45-50. */
 int sph_profile_begin(sph_handle h, uint32_t max_updates);
