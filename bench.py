@@ -2,7 +2,9 @@
 """Benchmark of the gfx950 SPH step (one MI355X).
 
 Default workload: uniform_1m -- 1,048,576 particles uniform in the lower half
-of a 16x8x16 box, spatial-hash table 2^21, h = 0.2, dt = 1/120 in 3 substeps.
+of a 16x8x16 box, h = 0.2, dt = 1/120 in 3 substeps, bucketed by the cell-grid
+key (--key reference: the reference's hash % T with the workload's T = 2^21;
+results are identical).
 A step is one sph_step(1/120, 3): three Update(dt/3) calls of the reference
 (physicsWorld.cc:39-111), i.e. three particle-substeps per particle.
 
@@ -11,7 +13,9 @@ resident in HBM when the timed region starts.  `roofline` is computed for the
 dominant kernel from its mean duration, measured live over the timed region
 with HIP events recorded on the simulation's own stream.  `cpu_baseline` times
 the CPU oracle (a restatement of the reference step, the reference's `par`
-loops run serially as they do without TBB) on a bounded sample.
+loops run serially as they do without TBB) on a bounded sample;
+`cpu_baseline_all_cores` the same with its per-particle loops on every core
+of the host's share (at most 16).
 """
 import argparse
 import json
@@ -107,19 +111,22 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(sc, substeps):
-    """Time the oracle (single thread) on `substeps` Update calls of the same input."""
+def cpu_baseline(sc, substeps, threads=1):
+    """Time the oracle on `substeps` Update calls of the same input, its
+    per-particle loops on `threads` host threads (OpenMP; the sort stays serial)."""
     import pyoracle as O
-    O.set_threads(1)
+    O.set_threads(threads)
     dt = float(np.float32(np.float32(sc.dt) / np.float32(sc.substeps)))
     op = O.from_sph(sc.params)
     t0 = time.perf_counter()
     O.update(op, dt, sc.pos, sc.vel, T=sc.table, substeps=substeps)
     el = time.perf_counter() - t0
+    O.set_threads(1)
     psub = sc.n * substeps / el
-    return {"value": psub / sc.substeps, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+    where = "1 host thread" if threads == 1 else f"{threads} host threads"
+    return {"value": psub / sc.substeps, "unit": "particle-steps/s", "cores": threads, "kind": "port",
             "sample": f"{substeps} Update(dt/3) calls of {sc.name} (n={sc.n}, table={sc.table}) "
-                      f"on 1 host thread, {el:.1f} s"}
+                      f"on {where}, {el:.1f} s"}
 
 
 def copy_calibration(mib):
@@ -367,7 +374,13 @@ def main():
                                                  seed=5)}
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
+        # the reference's PSTL `par` loops run serially without TBB (none in
+        # this image): one thread is its baseline; every core of this host's
+        # share (at most 16) is shown beside it
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_substeps)
+        threads = max(1, min(16, len(os.sched_getaffinity(0))))
+        if threads > 1:
+            out["cpu_baseline_all_cores"] = cpu_baseline(sc, args.cpu_substeps, threads)
     print(json.dumps(out), flush=True)
     return 0
 
