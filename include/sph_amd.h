@@ -207,7 +207,8 @@ int sph_count_nonfinite(sph_handle h, uint64_t *count);
 int sph_debug_spatial(sph_handle h, uint32_t *hash_id, uint32_t *key_id,
                       uint32_t *sorted_key, uint32_t *sorted_idx, uint32_t *start,
                       uint32_t *end);
-/* predictedPositions of the last update (physicsWorld.h:118), float4, w = 0. */
+/* predictedPositions of the last update (physicsWorld.h:118), float4, w = 0,
+ * particle order; SPH_ERR_STATE before the first update. */
 int sph_debug_predicted(sph_handle h, float *xp4);
 /* Neighbour sets of the last update as the reference's bucket walk sees them
  * (cc:331-363): for particle i, every j in the 27 cells whose hash matches and

@@ -86,3 +86,31 @@ def test_batch_autotune_keeps_state():
             pos, vel, _ = b.download(s)
             assert_bitexact(pos, sc.pos, f"scene {s} pos after autotune")
             assert_bitexact(vel, sc.vel, f"scene {s} vel after autotune")
+
+
+def test_many_scenes_grid_key():
+    """80 small scenes under the cell-grid key: their bucket tables add up to
+    more than 64 chunks of the count scan, so its look-back crosses several
+    64-chunk windows; every scene equals the oracle."""
+    rng = np.random.default_rng(11)
+    scenes = [W.batch_scene(int(n), float(h), float(mu), 100 + s) for s, (n, h, mu) in enumerate(
+        zip(rng.integers(200, 3000, 80), rng.uniform(0.08, 0.3, 80), rng.uniform(0.0, 0.2, 80)))]
+    announce("many scenes", len(scenes), sum(sc.n for sc in scenes))
+    with S.Batch([sc.params for sc in scenes], [sc.n for sc in scenes]) as b:
+        for s, sc in enumerate(scenes):
+            b.upload(s, sc.pos, sc.vel)
+        # each scene's grid bucket count, from the server's byte accounting
+        # (8 bytes per bucket): the count scan runs over more than 64 chunks
+        tables = [1 + (S.request_bytes(sc.params, sc.n) - S.request_bytes(sc.params, sc.n, 1)) // 8
+                  for sc in scenes]
+        assert sum(tables) > 64 * 2048, sum(tables)
+        for _ in range(2):
+            b.step(W.DT, W.SUBSTEPS)
+        got = [b.download(s) for s in range(len(scenes))]
+    dt = float(np.float32(np.float32(W.DT) / np.float32(W.SUBSTEPS)))
+    for s, sc in enumerate(scenes):
+        pos_o, vel_o, dens_o = O.update(orc_params(sc.params), dt, sc.pos, sc.vel, T=sc.n,
+                                        substeps=2 * W.SUBSTEPS)
+        assert_bitexact(got[s][0][:, :3], pos_o[:, :3], f"scene {s} pos")
+        assert_bitexact(got[s][1][:, :3], vel_o[:, :3], f"scene {s} vel")
+        assert_bitexact(got[s][2], dens_o, f"scene {s} dens")
