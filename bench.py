@@ -188,17 +188,32 @@ def bench_batch(steps, plain_steps, warmup, key="grid"):
         for sim in sims:
             sim.sync()
     el_p = time.perf_counter() - t0
+    # the same launches one scene after another: each scene's step finishes
+    # before the next scene's starts
+    t0 = time.perf_counter()
+    for _ in range(plain_steps):
+        for sim in sims:
+            sim.step(bs.dt, bs.substeps)
+    el_s = time.perf_counter() - t0
     for sim in sims:
         sim.close()
-    log(f"[bench] plain {plain_steps} steps in {el_p:.3f} s")
+    log(f"[bench] plain {plain_steps} steps in {el_p:.3f} s (serial {el_s:.3f} s)")
     v_b = total * steps / el_b
     v_p = total * plain_steps / el_p
+    v_s = total * plain_steps / el_s
     return {"workload": bs.name, "scenes": len(bs.scenes), "particles": total,
             "bucket_key": key, "value": v_b, "unit": "particle-steps/s", "ms_per_step": el_b * 1e3 / steps,
             "steps": steps,
             "plain_one_scene_per_launch": {"value": v_p, "ms_per_step": el_p * 1e3 / plain_steps,
-                                           "steps": plain_steps},
+                                           "steps": plain_steps,
+                                           "note": "every scene's own handle and stream, all enqueued "
+                                                   "before any is waited for (scenes overlap)"},
+            "plain_one_scene_at_a_time": {"value": v_s, "ms_per_step": el_s * 1e3 / plain_steps,
+                                          "steps": plain_steps,
+                                          "note": "the same launches, each scene's step finished "
+                                                  "before the next scene's starts"},
             "speedup_vs_plain": v_b / v_p,
+            "speedup_vs_one_at_a_time": v_b / v_s,
             "step_latency_ms": {"p50": float(np.percentile(lat, 50)),
                                 "p99": float(np.percentile(lat, 99)), "n": len(lat)},
             "stage_ms_per_update": {STAGES[i]: float(stage_ms[i]) for i in range(5)},
