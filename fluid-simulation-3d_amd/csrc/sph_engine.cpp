@@ -248,8 +248,6 @@ int Engine::create(const std::vector<SceneSpec> &scenes, uint32_t capacity,
                                  1.0f, 1.0f, 0.0f, 1.0f, 1.0f, 0.0f, 0.0f, 1.0f};
         memcpy(e->ed_h_->stops, stops, sizeof(stops));
     }
-    e->key_bits_ = bits_for(table_capacity - 1);
-    e->ed_h_->key_bits = e->key_bits_;
     work_key_layout(e->ed_h_->n_scenes, e->ed_h_);
     e->d_.multi = multi || scenes.size() > 1;
     e->grid_ = grid;

@@ -151,7 +151,6 @@ class Engine {
     EngineDesc *d_ed_ = nullptr;
     SceneDesc *d_sd_ = nullptr;
     uint32_t *d_scene_of_ = nullptr;
-    uint32_t key_bits_ = 1;
     uint32_t scene_cap_ = 1;
     bool have_tables_ = false;
     bool grid_ = false;  // cell-grid bucket key (else the reference's hash % table)

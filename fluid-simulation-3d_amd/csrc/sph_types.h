@@ -63,9 +63,8 @@ struct EngineDesc {
     uint32_t n_total;      // particles over all scenes
     uint32_t table_total;  // buckets over all scenes
     uint32_t n_scenes;
-    uint32_t key_bits;     // significant bits of the largest key (radix passes)
     uint32_t scene_shift;  // work-order key: scene index << scene_shift (32: one scene)
     uint32_t morton_drop;  // work-order key: Morton code >> morton_drop
-    uint32_t pad[2];
+    uint32_t pad[3];
     float stops[16];       // speed-colour gradient, Color1..Color4 RGBA (fluidSimCPU.h:26-29)
 };
