@@ -693,6 +693,33 @@ __device__ __forceinline__ void walk_run(const float4 *src, uint32_t base, const
     }
 }
 
+// The mask of a staged run of len <= 64 records src[0, len) when no float-hash
+// test is needed: bit o set when record o lies within h (d2 < thr, the walk's
+// expression).  Every lane starts at offset 0 and steps by KBG, so the offset
+// is one scalar for the wave: a batch's four results form a nibble that one
+// shift by that scalar places, with no per-record branch or bound check.
+// Bits at and past len (a batch reads up to KBG-1 records past the run, inside
+// the padded buffer) are cleared at the end.
+__device__ __forceinline__ uint64_t walk_bits(const float4 *src, uint32_t len, const float4 &pi, float thr)
+{
+    uint64_t mask = 0;
+#pragma unroll 1
+    for (uint32_t o = 0; o < len; o += KBG) {
+        float4 rr[KBG];
+#pragma unroll
+        for (int j = 0; j < KBG; ++j) rr[j] = src[o + j];
+        uint32_t nib = 0;
+#pragma unroll
+        for (int j = 0; j < KBG; ++j) {
+            const float ox = rr[j].x - pi.x, oy = rr[j].y - pi.y, oz = rr[j].z - pi.z;
+            const float d2 = ox * ox + oy * oy + oz * oz;  // glm::dot order
+            nib |= (d2 < thr ? 1u : 0u) << j;
+        }
+        mask |= (uint64_t)nib << o;
+    }
+    return len >= 64u ? mask : mask & ((1ull << len) - 1ull);
+}
+
 // Neighbour masks (grid key): for sorted slot k and row r, the run start
 // rb (base) and a 64-bit mask of the run's slots that passed (self
 // included); kept in the neighbour-row buffer as masks[9][n] (uint2) then
@@ -730,20 +757,19 @@ __device__ __forceinline__ void walk_grid(const SceneDesc &S, const float4 *__re
     // its bucket and every record of a row's run belongs to the row.
     const bool inner = clean && ux - 1u < S.gdim[0] - 2u && uy - 1u < gy - 2u && uz - 1u < gz - 2u;
     const bool nohash = __ballot(inner) == __ballot(1);
-    uint32_t wx[3], wy[3], wz[3];
+    uint32_t wz[3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        wx[d] = grid_wrap(ux + (uint32_t)(d - 1), S.gdim[0], S.gmod[0]);
-        wy[d] = grid_wrap(uy + (uint32_t)(d - 1), gy, S.gmod[1]);
-        wz[d] = grid_wrap(uz + (uint32_t)(d - 1), gz, S.gmod[2]);
-    }
+    for (int d = 0; d < 3; ++d) wz[d] = grid_wrap(uz + (uint32_t)(d - 1), gz, S.gmod[2]);
     const bool run = wz[1] == wz[0] + 1u && wz[2] == wz[1] + 1u;
     const uint32_t h0 = hash_cell(cx, cy, cz);
     // bucket bounds of the next row: the run [start of its first bucket, end of
-    // its last) when its buckets are consecutive, else each bucket's
+    // its last) when its buckets are consecutive, else each bucket's.  The
+    // row's x and y cells are wrapped when fetched (fewer live registers).
     uint32_t ns[3] = {0u, 0u, 0u}, ne[3] = {0u, 0u, 0u};
     auto fetch = [&](int r) {
-        const uint32_t base = key_off + (wx[r / 3] * gy + wy[r % 3]) * gz;
+        const uint32_t wx = grid_wrap(ux + (uint32_t)(r / 3 - 1), S.gdim[0], S.gmod[0]);
+        const uint32_t wy = grid_wrap(uy + (uint32_t)(r % 3 - 1), gy, S.gmod[1]);
+        const uint32_t base = key_off + (wx * gy + wy) * gz;
         if (run) {
             ns[0] = start[base + wz[0]];
             ne[2] = endt[base + wz[2]];
@@ -792,7 +818,9 @@ __device__ __forceinline__ void walk_grid(const SceneDesc &S, const float4 *__re
             auto inline_body = [&](uint32_t m, float ox, float oy, float oz, float d2, const float4 &a) {
                 if (!defer) body(m, ox, oy, oz, d2, a);
             };
-            if (staged && nohash)
+            if (DEFER && !SKIP_SELF && !AUX && staged && nohash && __ballot(defer) == __ballot(1))
+                mask = walk_bits(stage + (rb - lo), re - rb, pi, thr);
+            else if (staged && nohash)
                 walk_run<true, SKIP_SELF, AUX, MASKS || DEFER, false>(stage, lo, xps, aux, k, pi, thr, rb, re,
                                                                      f0, f1, f2, mask, inline_body);
             else if (staged)
