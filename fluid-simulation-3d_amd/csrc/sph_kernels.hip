@@ -878,10 +878,15 @@ __device__ __forceinline__ void walk_grid(const SceneDesc &S, const float4 *__re
 // Consumer side of the neighbour masks: body(m, ox, oy, oz, d2, aux[m]) for
 // every recorded slot m (m != k when SKIP_SELF), rows in order and slots
 // ascending within a row -- the order of the walk that recorded them.  d2 is
-// recomputed with the walk's expression.  The nine rows' masks and bases are
-// loaded at once into the wave's LDS table `tab` (rows x 64 lanes, 3 words
-// each), so stepping to the next row costs an LDS read; NB neighbours'
-// records are loaded per round.
+// recomputed with the walk's expression.  The rows' masks and bases are loaded
+// at once into the wave's LDS table `tab` (rows x 64 lanes, 3 words each),
+// empty rows left out, so every table row holds at least one slot.  With
+// SKIP_SELF the particle's own slot k is cleared from row 4 (dx = dy = 0), the
+// only row whose mask can hold it: it is a slot of the particle's own cell.
+// Slots are taken from two registers, the current row's remaining bits and
+// the next row's, without branches; NB are taken per round (a round can end
+// short only when two rows run out within it), their records loaded together,
+// and the table refills the two registers once per round.
 constexpr uint32_t kMaskTabWords = 9 * 64 * 3;  // per wave
 
 template <int NB, bool SKIP_SELF, bool AUX, typename Body>
@@ -896,49 +901,66 @@ __device__ __forceinline__ void gather_masks(const float4 *__restrict__ xps,
     const uint32_t lane = threadIdx.x & 63u;
     uint2 *tm = reinterpret_cast<uint2 *>(tab);  // [9][64] masks
     uint32_t *tb = tab + 9 * 64 * 2;             // [9][64] bases
+    uint32_t rows = 0;                           // non-empty rows in the table
 #pragma unroll
     for (int r = 0; r < 9; ++r) {
-        tm[r * 64 + lane] = mk[(size_t)r * n + k];
-        tb[r * 64 + lane] = bs[(size_t)r * n + k];
+        uint2 w = mk[(size_t)r * n + k];
+        const uint32_t b = bs[(size_t)r * n + k];
+        if (SKIP_SELF && r == 4) {
+            const uint32_t o = k - b;
+            w.x &= o < 32u ? ~(1u << o) : ~0u;
+            w.y &= o - 32u < 32u ? ~(1u << (o - 32u)) : ~0u;
+        }
+        tm[rows * 64 + lane] = w;
+        tb[rows * 64 + lane] = b;
+        rows += (w.x | w.y) != 0u ? 1u : 0u;
     }
-    uint64_t cur = 0;
-    uint32_t cb = 0;
-    uint32_t r = 0;  // next row to take
-    auto take = [&](uint32_t &m) -> bool {
-        while (cur == 0) {
-            if (r == 9) return false;
+    uint64_t cur = 0, nxt = 0;  // remaining bits of the current and the next row
+    uint32_t cb = 0, nb = 0;    // their run starts
+    uint32_t r = 0;             // next table row
+    auto refill = [&]() {
+        // cur == 0 implies nxt == 0: a row is moved to cur only from nxt
+        if (cur == 0 && r < rows) {
             const uint2 w = tm[r * 64 + lane];
             cur = ((uint64_t)w.y << 32) | w.x;
             cb = tb[r * 64 + lane];
             ++r;
         }
-        m = cb + (uint32_t)__builtin_ctzll(cur);
-        cur &= cur - 1;
-        return true;
+        if (nxt == 0 && r < rows) {
+            const uint2 w = tm[r * 64 + lane];
+            nxt = ((uint64_t)w.y << 32) | w.x;
+            nb = tb[r * 64 + lane];
+            ++r;
+        }
     };
+    refill();
 #pragma unroll 1
-    for (;;) {
+    while (cur != 0) {
         uint32_t m[NB];
         bool v[NB];
 #pragma unroll
-        for (int q = 0; q < NB; ++q) v[q] = take(m[q]) && !(SKIP_SELF && m[q] == k);
+        for (int q = 0; q < NB; ++q) {
+            v[q] = cur != 0;
+            m[q] = v[q] ? cb + (uint32_t)__builtin_ctzll(cur) : k;
+            cur &= cur - 1;
+            const bool e = cur == 0;
+            cur = e ? nxt : cur;
+            cb = e ? nb : cb;
+            nxt = e ? 0 : nxt;
+        }
         float4 rr[NB], a[NB];
 #pragma unroll
         for (int q = 0; q < NB; ++q) {
-            rr[q] = pi;
-            a[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (v[q]) {
-                rr[q] = xps[m[q]];
-                if (AUX) a[q] = aux[m[q]];
-            }
+            rr[q] = xps[m[q]];
+            a[q] = AUX ? aux[m[q]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
+        refill();
 #pragma unroll
         for (int q = 0; q < NB; ++q)
             if (v[q]) {
                 const float ox = rr[q].x - pi.x, oy = rr[q].y - pi.y, oz = rr[q].z - pi.z;
                 body(m[q], ox, oy, oz, ox * ox + oy * oy + oz * oz, a[q]);
             }
-        if (cur == 0 && r == 9) break;  // every recorded slot has been taken
     }
 }
 
