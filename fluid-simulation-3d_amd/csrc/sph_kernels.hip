@@ -806,7 +806,16 @@ __device__ __forceinline__ void walk_grid(const SceneDesc &S, const float4 *__re
                         (uint32_t)(act >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0u));
                     const uint32_t cnt = (uint32_t)__builtin_popcountll(act);
                     __builtin_amdgcn_wave_barrier();  // the previous row's LDS reads are issued
-                    for (uint32_t t = rank; t < hi - lo; t += cnt) stage[t] = xps[lo + t];
+                    // two records per lane in flight before either is stored
+                    // (past the range the second is the first again)
+                    const uint32_t len = hi - lo;
+                    for (uint32_t t = rank; t < len; t += 2 * cnt) {
+                        const uint32_t u = t + cnt < len ? t + cnt : t;
+                        const float4 a0 = xps[lo + t];
+                        const float4 a1 = xps[lo + u];
+                        stage[t] = a0;
+                        stage[u] = a1;
+                    }
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1348,7 +1357,7 @@ __global__ __launch_bounds__(256, MULTI ? 4 : 5) void k_viscosity_integrate(SPH_
         fz += (vj.z - vi.z) * w;
     };
     if (NBR && GRID) {
-        if (ncnt[k]) gather_masks<8, true, true>(xps, vs2, nbr, ed->n_total, k, pi, mtab, body);
+        if (ncnt[k]) gather_masks<4, true, true>(xps, vs2, nbr, ed->n_total, k, pi, mtab, body);
         else gather<GRID, false, true, true>(S, xps, vs2, start, endt, lst, stage, clean, k, pi, body);
     } else if (NBR) {
         gather_row(S, xps, vs2, start, endt, nbr + (size_t)i * kNbrMax, ncnt[i], k, pi, body);
