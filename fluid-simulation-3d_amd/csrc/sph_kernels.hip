@@ -724,7 +724,10 @@ __device__ __forceinline__ uint64_t walk_bits(const float4 *src, uint32_t len, c
 // rb (base) and a 64-bit mask of the run's slots that passed (self
 // included); kept in the neighbour-row buffer as masks[9][n] (uint2) then
 // bases[9][n]; ok[k] = 0 when a row could not be recorded (a run longer than
-// 64 slots, or a seam row), and the consumer then walks that particle.
+// 64 slots, or a seam row), and the consumer then walks that particle; else
+// ok[k] = kMasksOk | the bit set of the rows whose mask is not empty.
+constexpr uint32_t kMasksOk = 1u << 9;
+
 struct MaskOut {
     uint32_t *buf;  // neighbour-row buffer (>= 27 n words)
     uint32_t *ok;   // [n]
@@ -736,8 +739,12 @@ struct MaskOut {
 // in ascending order (the walk order), reading the records from the wave's
 // LDS copy of the row when it was staged.  The body then costs one pass per
 // neighbour instead of one per tested candidate batch.
-template <bool SKIP_SELF, bool AUX, bool MASKS = false, bool DEFER = false, typename Body>
-__device__ __forceinline__ void walk_grid(const SceneDesc &S, const float4 *__restrict__ xps,
+// MASK_ONLY (with MASKS): the walk records the masks and calls no body; seam
+// rows are not walked (they make the particle's masks unusable anyway).
+// Returns the value written to ok[k] (0 without MASKS).
+template <bool SKIP_SELF, bool AUX, bool MASKS = false, bool DEFER = false, bool MASK_ONLY = false,
+          typename Body>
+__device__ __forceinline__ uint32_t walk_grid(const SceneDesc &S, const float4 *__restrict__ xps,
                                           const float4 *__restrict__ aux,
                                           const uint32_t *__restrict__ start,
                                           const uint32_t *__restrict__ endt, uint32_t k,
@@ -748,6 +755,7 @@ __device__ __forceinline__ void walk_grid(const SceneDesc &S, const float4 *__re
     const float thr = S.thr;
     const uint32_t gy = S.gdim[1], gz = S.gdim[2], key_off = S.key_off;
     bool masks_ok = true;
+    uint32_t rowbits = 0;  // rows whose mask is not empty
     const uint32_t ux = (uint32_t)cx - (uint32_t)S.cell_lo[0];
     const uint32_t uy = (uint32_t)cy - (uint32_t)S.cell_lo[1];
     const uint32_t uz = (uint32_t)cz - (uint32_t)S.cell_lo[2];
@@ -825,9 +833,10 @@ __device__ __forceinline__ void walk_grid(const SceneDesc &S, const float4 *__re
             uint64_t mask = 0;
             const bool defer = DEFER && re - rb <= 64u;
             auto inline_body = [&](uint32_t m, float ox, float oy, float oz, float d2, const float4 &a) {
-                if (!defer) body(m, ox, oy, oz, d2, a);
+                if (!MASK_ONLY && !defer) body(m, ox, oy, oz, d2, a);
             };
-            if (DEFER && !SKIP_SELF && !AUX && staged && nohash && __ballot(defer) == __ballot(1))
+            if ((DEFER || MASK_ONLY) && !SKIP_SELF && !AUX && staged && nohash &&
+                __ballot(re - rb <= 64u) == __ballot(1))
                 mask = walk_bits(stage + (rb - lo), re - rb, pi, thr);
             else if (staged && nohash)
                 walk_run<true, SKIP_SELF, AUX, MASKS || DEFER, false>(stage, lo, xps, aux, k, pi, thr, rb, re,
@@ -863,11 +872,12 @@ __device__ __forceinline__ void walk_grid(const SceneDesc &S, const float4 *__re
             }
             if (MASKS) {
                 masks_ok = masks_ok && re - rb <= 64u;
+                rowbits |= (mask != 0 ? 1u : 0u) << r;
                 reinterpret_cast<uint2 *>(mo.buf)[(size_t)r * mo.n + k] =
                     make_uint2((uint32_t)mask, (uint32_t)(mask >> 32));
                 mo.buf[(size_t)(18 + r) * mo.n + k] = rb;
             }
-        } else {
+        } else if (!MASK_ONLY) {
             masks_ok = false;
             // wrap seam: the three buckets one by one
 #pragma unroll 1
@@ -879,68 +889,34 @@ __device__ __forceinline__ void walk_grid(const SceneDesc &S, const float4 *__re
                 walk_run<false, SKIP_SELF, AUX, false, true>(xps, 0, xps, aux, k, pi, thr, b, e, fq, fq, fq, unused,
                                                              body);
             }
+        } else {
+            masks_ok = false;
         }
     }
-    if (MASKS) mo.ok[k] = masks_ok ? 1u : 0u;
+    const uint32_t ok = masks_ok ? kMasksOk | rowbits : 0u;
+    if (MASKS) mo.ok[k] = ok;
+    return MASKS ? ok : 0u;
 }
 
 // Consumer side of the neighbour masks: body(m, ox, oy, oz, d2, aux[m]) for
-// every recorded slot m (m != k when SKIP_SELF), rows in order and slots
-// ascending within a row -- the order of the walk that recorded them.  d2 is
-// recomputed with the walk's expression.  The rows' masks and bases are loaded
-// at once into the wave's LDS table `tab` (rows x 64 lanes, 3 words each),
-// empty rows left out, so every table row holds at least one slot.  With
-// SKIP_SELF the particle's own slot k is cleared from row 4 (dx = dy = 0), the
-// only row whose mask can hold it: it is a slot of the particle's own cell.
-// Slots are taken from two registers, the current row's remaining bits and
-// the next row's, without branches; NB are taken per round (a round can end
-// short only when two rows run out within it), their records loaded together,
-// and the table refills the two registers once per round.
-constexpr uint32_t kMaskTabWords = 9 * 64 * 3;  // per wave
-
-template <int NB, bool SKIP_SELF, bool AUX, typename Body>
-__device__ __forceinline__ void gather_masks(const float4 *__restrict__ xps,
-                                             const float4 *__restrict__ aux,
-                                             const uint32_t *__restrict__ buf, uint32_t n,
-                                             uint32_t k, const float4 &pi, uint32_t *tab,
-                                             Body &&body)
+// every recorded slot m, rows in order and slots ascending within a row --
+// the order of the walk that recorded them.  d2 is recomputed with the walk's
+// expression.  next_row(mask, base) gives the next row that holds a slot, or
+// leaves mask = 0 when none is left.  Slots are taken from two registers, the
+// current row's remaining bits and the next row's, without branches; NB are
+// taken per round (a round can end short only when two rows run out within
+// it), their records loaded together, and the two registers refilled once
+// per round.
+template <int NB, bool AUX, typename NextRow, typename Body>
+__device__ __forceinline__ void take_masks(const float4 *__restrict__ xps, const float4 *__restrict__ aux,
+                                           uint32_t k, const float4 &pi, NextRow &&next_row, Body &&body)
 {
-    const uint2 *mk = reinterpret_cast<const uint2 *>(buf);
-    const uint32_t *bs = buf + 18 * (size_t)n;
-    const uint32_t lane = threadIdx.x & 63u;
-    uint2 *tm = reinterpret_cast<uint2 *>(tab);  // [9][64] masks
-    uint32_t *tb = tab + 9 * 64 * 2;             // [9][64] bases
-    uint32_t rows = 0;                           // non-empty rows in the table
-#pragma unroll
-    for (int r = 0; r < 9; ++r) {
-        uint2 w = mk[(size_t)r * n + k];
-        const uint32_t b = bs[(size_t)r * n + k];
-        if (SKIP_SELF && r == 4) {
-            const uint32_t o = k - b;
-            w.x &= o < 32u ? ~(1u << o) : ~0u;
-            w.y &= o - 32u < 32u ? ~(1u << (o - 32u)) : ~0u;
-        }
-        tm[rows * 64 + lane] = w;
-        tb[rows * 64 + lane] = b;
-        rows += (w.x | w.y) != 0u ? 1u : 0u;
-    }
     uint64_t cur = 0, nxt = 0;  // remaining bits of the current and the next row
     uint32_t cb = 0, nb = 0;    // their run starts
-    uint32_t r = 0;             // next table row
     auto refill = [&]() {
         // cur == 0 implies nxt == 0: a row is moved to cur only from nxt
-        if (cur == 0 && r < rows) {
-            const uint2 w = tm[r * 64 + lane];
-            cur = ((uint64_t)w.y << 32) | w.x;
-            cb = tb[r * 64 + lane];
-            ++r;
-        }
-        if (nxt == 0 && r < rows) {
-            const uint2 w = tm[r * 64 + lane];
-            nxt = ((uint64_t)w.y << 32) | w.x;
-            nb = tb[r * 64 + lane];
-            ++r;
-        }
+        if (cur == 0) next_row(cur, cb);
+        if (nxt == 0) next_row(nxt, nb);
     };
     refill();
 #pragma unroll 1
@@ -973,8 +949,82 @@ __device__ __forceinline__ void gather_masks(const float4 *__restrict__ xps,
     }
 }
 
-template <bool GRID, bool SKIP_SELF, bool AUX, bool MASKS = false, bool DEFER = false, typename Body>
-__device__ __forceinline__ void walk(const SceneDesc &S, const float4 *__restrict__ xps,
+// The mask consumers (pressure, viscosity).  The rows' masks and bases are
+// loaded at once into the wave's LDS table `tab` (rows x 64 lanes, 3 words
+// each), empty rows left out, so every table row holds at least one slot.
+// With SKIP_SELF the particle's own slot k is cleared from row 4 (dx = dy =
+// 0), the only row whose mask can hold it: it is a slot of the particle's own
+// cell.
+constexpr uint32_t kMaskTabWords = 9 * 64 * 3;  // per wave
+
+template <int NB, bool SKIP_SELF, bool AUX, typename Body>
+__device__ __forceinline__ void gather_masks(const float4 *__restrict__ xps,
+                                             const float4 *__restrict__ aux,
+                                             const uint32_t *__restrict__ buf, uint32_t n,
+                                             uint32_t k, const float4 &pi, uint32_t *tab,
+                                             Body &&body)
+{
+    const uint2 *mk = reinterpret_cast<const uint2 *>(buf);
+    const uint32_t *bs = buf + 18 * (size_t)n;
+    const uint32_t lane = threadIdx.x & 63u;
+    uint2 *tm = reinterpret_cast<uint2 *>(tab);  // [9][64] masks
+    uint32_t *tb = tab + 9 * 64 * 2;             // [9][64] bases
+    uint32_t rows = 0;                           // non-empty rows in the table
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        uint2 w = mk[(size_t)r * n + k];
+        const uint32_t b = bs[(size_t)r * n + k];
+        if (SKIP_SELF && r == 4) {
+            const uint32_t o = k - b;
+            w.x &= o < 32u ? ~(1u << o) : ~0u;
+            w.y &= o - 32u < 32u ? ~(1u << (o - 32u)) : ~0u;
+        }
+        tm[rows * 64 + lane] = w;
+        tb[rows * 64 + lane] = b;
+        rows += (w.x | w.y) != 0u ? 1u : 0u;
+    }
+    uint32_t r = 0;  // next table row
+    take_masks<NB, AUX>(xps, aux, k, pi,
+                        [&](uint64_t &mask, uint32_t &base) {
+                            if (r < rows) {
+                                const uint2 w = tm[r * 64 + lane];
+                                mask = ((uint64_t)w.y << 32) | w.x;
+                                base = tb[r * 64 + lane];
+                                ++r;
+                            }
+                        },
+                        body);
+}
+
+// Density over the masks it has just recorded (MASK_ONLY walk): the rows are
+// read back from the neighbour-row buffer (written by this work-item, so
+// still in cache) as they are needed, the non-empty ones picked by `rowbits`
+// (bit r: row r holds a slot).  The body passes then follow the particle's
+// total neighbour count rather than each row's largest count in the wave.
+template <int NB, typename Body>
+__device__ __forceinline__ void gather_own_masks(const float4 *__restrict__ xps,
+                                                 const uint32_t *__restrict__ buf, uint32_t n,
+                                                 uint32_t k, const float4 &pi, uint32_t rowbits,
+                                                 Body &&body)
+{
+    const uint2 *mk = reinterpret_cast<const uint2 *>(buf);
+    const uint32_t *bs = buf + 18 * (size_t)n;
+    take_masks<NB, false>(xps, nullptr, k, pi,
+                          [&](uint64_t &mask, uint32_t &base) {
+                              if (rowbits != 0) {
+                                  const uint32_t r = (uint32_t)__builtin_ctz(rowbits);
+                                  rowbits &= rowbits - 1;
+                                  const uint2 w = mk[(size_t)r * n + k];
+                                  mask = ((uint64_t)w.y << 32) | w.x;
+                                  base = bs[(size_t)r * n + k];
+                              }
+                          },
+                          body);
+}
+
+template <bool GRID, bool SKIP_SELF, bool AUX, bool MASKS = false, bool DEFER = false,
+          bool MASK_ONLY = false, typename Body>
+__device__ __forceinline__ uint32_t walk(const SceneDesc &S, const float4 *__restrict__ xps,
                                      const float4 *__restrict__ aux,
                                      const uint32_t *__restrict__ start,
                                      const uint32_t *__restrict__ endt, uint32_t k,
@@ -982,8 +1032,10 @@ __device__ __forceinline__ void walk(const SceneDesc &S, const float4 *__restric
                                      const MaskOut &mo = MaskOut{nullptr, nullptr, 0})
 {
     if constexpr (GRID)
-        walk_grid<SKIP_SELF, AUX, MASKS, DEFER>(S, xps, aux, start, endt, k, pi, stage, clean, body, mo);
+        return walk_grid<SKIP_SELF, AUX, MASKS, DEFER, MASK_ONLY>(S, xps, aux, start, endt, k, pi, stage, clean,
+                                                                 body, mo);
     else walk_hash<SKIP_SELF, AUX>(S, xps, aux, start, endt, k, pi, body);
+    return 0u;
 }
 
 // Gather kernels (density, pressure, viscosity), in result-identical
@@ -1185,11 +1237,18 @@ __global__ __launch_bounds__(256, MULTI ? 6 : 8) void k_density(SPH_GATHER_PARAM
             ++c;
         }
     };
-    // MASKS: the walk also records the neighbour masks.  Grid key without
-    // the LDS list: the body runs over each row's mask (DEFER).
-    if (GRID && !LIST)
-        walk<GRID, false, false, MASKS, true>(S, xps, nullptr, start, endt, k, pi, stage, clean, body,
-                                              MaskOut{nbr, ncnt, ed->n_total});
+    // MASKS: the walk records the neighbour masks and the bodies then run over
+    // all of them; a particle whose masks are unusable (a run over 64 slots,
+    // a seam row) walks again with the body.  Grid key without masks or the
+    // LDS list: the body runs over each row's mask as the walk goes (DEFER).
+    if (GRID && !LIST && MASKS) {
+        const uint32_t ok = walk<GRID, false, false, true, false, true>(
+            S, xps, nullptr, start, endt, k, pi, stage, clean,
+            [](uint32_t, float, float, float, float, const float4 &) {}, MaskOut{nbr, ncnt, ed->n_total});
+        if (ok) gather_own_masks<4>(xps, nbr, ed->n_total, k, pi, ok & (kMasksOk - 1u), body);
+        else walk<GRID, false, false>(S, xps, nullptr, start, endt, k, pi, stage, clean, body);
+    } else if (GRID && !LIST)
+        walk<GRID, false, false, false, true>(S, xps, nullptr, start, endt, k, pi, stage, clean, body);
     else
         gather<GRID, LIST, false, false, MASKS>(S, xps, nullptr, start, endt, lst, stage, clean, k, pi, body,
                                                 MaskOut{nbr, ncnt, ed->n_total});
