@@ -902,21 +902,40 @@ __device__ __forceinline__ uint32_t walk_grid(const SceneDesc &S, const float4 *
 // every recorded slot m, rows in order and slots ascending within a row --
 // the order of the walk that recorded them.  d2 is recomputed with the walk's
 // expression.  next_row(mask, base) gives the next row that holds a slot, or
-// leaves mask = 0 when none is left.  Slots are taken from two registers, the
-// current row's remaining bits and the next row's, without branches; NB are
-// taken per round (a round can end short only when two rows run out within
-// it), their records loaded together, and the two registers refilled once
-// per round.
-template <int NB, bool AUX, typename NextRow, typename Body>
+// leaves mask = 0 when none is left.  A row is taken as its two 32-slot halves
+// (empty halves skipped), and slots are taken from two registers, the current
+// half's remaining bits and the next half's, with 32-bit operations and no
+// branches; NB are taken per round (a round can end short only when two
+// halves run out within it), their records loaded together, and the two
+// registers refilled once per round.  AUXN: the components of aux[m] the
+// body reads (0, 3 or 4); only those are loaded.
+template <int NB, int AUXN, typename NextRow, typename Body>
 __device__ __forceinline__ void take_masks(const float4 *__restrict__ xps, const float4 *__restrict__ aux,
                                            uint32_t k, const float4 &pi, NextRow &&next_row, Body &&body)
 {
-    uint64_t cur = 0, nxt = 0;  // remaining bits of the current and the next row
-    uint32_t cb = 0, nb = 0;    // their run starts
+    uint32_t cur = 0, nxt = 0;  // remaining bits of the current and the next half
+    uint32_t cb = 0, nb = 0;    // their first slots
+    uint32_t hi = 0, hb = 0;    // upper half of the row last taken, not yet used
+    auto next_half = [&](uint32_t &m, uint32_t &b) {
+        if (hi != 0) {
+            m = hi;
+            b = hb;
+            hi = 0;
+            return;
+        }
+        uint64_t row = 0;
+        uint32_t base = 0;
+        next_row(row, base);
+        const uint32_t lo32 = (uint32_t)row, hi32 = (uint32_t)(row >> 32);
+        m = lo32 != 0 ? lo32 : hi32;
+        b = lo32 != 0 ? base : base + 32u;
+        hi = lo32 != 0 ? hi32 : 0u;
+        hb = base + 32u;
+    };
     auto refill = [&]() {
-        // cur == 0 implies nxt == 0: a row is moved to cur only from nxt
-        if (cur == 0) next_row(cur, cb);
-        if (nxt == 0) next_row(nxt, nb);
+        // cur == 0 implies nxt == 0: a half is moved to cur only from nxt
+        if (cur == 0) next_half(cur, cb);
+        if (nxt == 0) next_half(nxt, nb);
     };
     refill();
 #pragma unroll 1
@@ -926,18 +945,25 @@ __device__ __forceinline__ void take_masks(const float4 *__restrict__ xps, const
 #pragma unroll
         for (int q = 0; q < NB; ++q) {
             v[q] = cur != 0;
-            m[q] = v[q] ? cb + (uint32_t)__builtin_ctzll(cur) : k;
+            m[q] = v[q] ? cb + (uint32_t)__builtin_ctz(cur) : k;
             cur &= cur - 1;
             const bool e = cur == 0;
             cur = e ? nxt : cur;
             cb = e ? nb : cb;
-            nxt = e ? 0 : nxt;
+            nxt = e ? 0u : nxt;
         }
         float4 rr[NB], a[NB];
 #pragma unroll
         for (int q = 0; q < NB; ++q) {
             rr[q] = xps[m[q]];
-            a[q] = AUX ? aux[m[q]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (AUXN == 4) {
+                a[q] = aux[m[q]];
+            } else if (AUXN == 3) {
+                const float3 t = *reinterpret_cast<const float3 *>(aux + m[q]);
+                a[q] = make_float4(t.x, t.y, t.z, 0.0f);
+            } else {
+                a[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
         }
         refill();
 #pragma unroll
@@ -957,7 +983,7 @@ __device__ __forceinline__ void take_masks(const float4 *__restrict__ xps, const
 // cell.
 constexpr uint32_t kMaskTabWords = 9 * 64 * 3;  // per wave
 
-template <int NB, bool SKIP_SELF, bool AUX, typename Body>
+template <int NB, bool SKIP_SELF, int AUXN, typename Body>
 __device__ __forceinline__ void gather_masks(const float4 *__restrict__ xps,
                                              const float4 *__restrict__ aux,
                                              const uint32_t *__restrict__ buf, uint32_t n,
@@ -984,7 +1010,7 @@ __device__ __forceinline__ void gather_masks(const float4 *__restrict__ xps,
         rows += (w.x | w.y) != 0u ? 1u : 0u;
     }
     uint32_t r = 0;  // next table row
-    take_masks<NB, AUX>(xps, aux, k, pi,
+    take_masks<NB, AUXN>(xps, aux, k, pi,
                         [&](uint64_t &mask, uint32_t &base) {
                             if (r < rows) {
                                 const uint2 w = tm[r * 64 + lane];
@@ -1009,7 +1035,7 @@ __device__ __forceinline__ void gather_own_masks(const float4 *__restrict__ xps,
 {
     const uint2 *mk = reinterpret_cast<const uint2 *>(buf);
     const uint32_t *bs = buf + 18 * (size_t)n;
-    take_masks<NB, false>(xps, nullptr, k, pi,
+    take_masks<NB, 0>(xps, nullptr, k, pi,
                           [&](uint64_t &mask, uint32_t &base) {
                               if (rowbits != 0) {
                                   const uint32_t r = (uint32_t)__builtin_ctz(rowbits);
@@ -1368,7 +1394,7 @@ __global__ __launch_bounds__(256) void k_pressure(SPH_GATHER_PARAMS, const float
         }
     };
     if (NBR && GRID) {
-        if (ncnt[k]) gather_masks<KB, true, true>(xps, dens, nbr, ed->n_total, k, pi, mtab, body);
+        if (ncnt[k]) gather_masks<KB, true, 4>(xps, dens, nbr, ed->n_total, k, pi, mtab, body);
         else gather<GRID, false, true, true>(S, xps, dens, start, endt, lst, stage, clean, k, pi, body);
     } else if (NBR) {
         gather_row(S, xps, dens, start, endt, nbr + (size_t)i * kNbrMax, ncnt[i], k, pi, body);
@@ -1416,7 +1442,7 @@ __global__ __launch_bounds__(256, MULTI ? 4 : 5) void k_viscosity_integrate(SPH_
         fz += (vj.z - vi.z) * w;
     };
     if (NBR && GRID) {
-        if (ncnt[k]) gather_masks<4, true, true>(xps, vs2, nbr, ed->n_total, k, pi, mtab, body);
+        if (ncnt[k]) gather_masks<4, true, 3>(xps, vs2, nbr, ed->n_total, k, pi, mtab, body);
         else gather<GRID, false, true, true>(S, xps, vs2, start, endt, lst, stage, clean, k, pi, body);
     } else if (NBR) {
         gather_row(S, xps, vs2, start, endt, nbr + (size_t)i * kNbrMax, ncnt[i], k, pi, body);
