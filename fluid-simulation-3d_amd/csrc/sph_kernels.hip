@@ -1023,10 +1023,11 @@ __device__ __forceinline__ void gather_masks(const float4 *__restrict__ xps,
 }
 
 // Density over the masks it has just recorded (MASK_ONLY walk): the rows are
-// read back from the neighbour-row buffer (written by this work-item, so
-// still in cache) as they are needed, the non-empty ones picked by `rowbits`
-// (bit r: row r holds a slot).  The body passes then follow the particle's
-// total neighbour count rather than each row's largest count in the wave.
+// read back from the neighbour-row buffer, written by this work-item during
+// the walk, as they are needed, the non-empty ones picked by `rowbits` (bit
+// r: row r holds a slot).  The wave's body passes then follow its largest
+// total neighbour count rather than the sum over rows of each row's largest
+// count.
 template <int NB, typename Body>
 __device__ __forceinline__ void gather_own_masks(const float4 *__restrict__ xps,
                                                  const uint32_t *__restrict__ buf, uint32_t n,
