@@ -107,6 +107,25 @@ def measured_traffic(workload, kernel):
     return None if k is None else float(k["traffic_bytes"])
 
 
+def gather_fractions(sc, variants, grid, per):
+    """HBM and VALU fractions of each gather pass (density, pressure,
+    viscosity+integrate) at its live duration: algorithmic bytes / duration /
+    8 TB/s, the committed measured traffic / duration / 8 TB/s, and the VALU
+    issue share from the committed instruction count."""
+    out = {}
+    for stage in ("density", "pressure", "viscosity_integrate"):
+        kern = kernel_name(stage, variants, grid)
+        ms = per[stage]
+        traffic = measured_traffic(sc.name, kern)
+        valu = valu_issue(sc.name, kern, ms)
+        out[stage] = {"kernel": kern, "ms": ms,
+                      "alg_hbm_frac": ALG_BYTES[stage] * sc.n / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                      "traffic_hbm_frac": None if traffic is None
+                      else traffic / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                      "valu_issue_frac": None if valu is None else valu["issue_frac"]}
+    return out
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -375,6 +394,7 @@ def main():
                      "alg_bytes_per_launch": ALG_BYTES[dom] * sc.n,
                      "alg_bytes_per_particle": ALG_BYTES[dom]},
         "valu": valu_issue(sc.name, dom_kernel, per[dom]),
+        "gather_kernels": gather_fractions(sc, variants, key_grid, per),
         "nonfinite_components": nonfinite,
     }
     if not args.no_batch:
