@@ -29,12 +29,12 @@ int set_last_error(int code, const std::string &msg);  // sph_capi.cpp
 
 namespace {
 
-// Device bytes per particle of an engine: eight float4 arrays (pos, vel,
-// xp, ps, xps, vs, vs2, dens), thirteen u32 words (bucket key and ticket, the
-// (slot, index) pair of the bucket order, resident and sorted ids, two
-// work-order key/value pairs, rank, scene map, row count), the 64-entry
-// neighbour row and two float4 staging arrays.
-constexpr uint64_t kBytesPerParticle = 8 * 16 + 13 * 4 + 64 * 4 + 2 * 16;
+// Device bytes per particle of an engine: seven float4 arrays (pos, vel, ps,
+// xps, vs, vs2, dens), thirteen u32 words (bucket key and ticket, the (slot,
+// index) pair of the bucket order, resident and sorted ids, two work-order
+// key/value pairs, rank, scene map, row count), the 64-entry neighbour row
+// and two float4 staging arrays.
+constexpr uint64_t kBytesPerParticle = 7 * 16 + 13 * 4 + 64 * 4 + 2 * 16;
 constexpr uint64_t kBytesPerBucket = 8;  // bucket bounds + counts
 
 double now_s()
