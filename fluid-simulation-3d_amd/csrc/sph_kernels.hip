@@ -1024,10 +1024,10 @@ __device__ __forceinline__ void gather_masks(const float4 *__restrict__ xps,
 
 // Density over the masks it has just recorded (MASK_ONLY walk): the rows are
 // read back from the neighbour-row buffer, written by this work-item during
-// the walk, as they are needed, the non-empty ones picked by `rowbits` (bit
-// r: row r holds a slot).  The wave's body passes then follow its largest
-// total neighbour count rather than the sum over rows of each row's largest
-// count.
+// the walk, one row ahead of the one being taken, the non-empty ones picked
+// by `rowbits` (bit r: row r holds a slot).  The wave's body passes then
+// follow its largest total neighbour count rather than the sum over rows of
+// each row's largest count.
 template <int NB, typename Body>
 __device__ __forceinline__ void gather_own_masks(const float4 *__restrict__ xps,
                                                  const uint32_t *__restrict__ buf, uint32_t n,
@@ -1036,17 +1036,26 @@ __device__ __forceinline__ void gather_own_masks(const float4 *__restrict__ xps,
 {
     const uint2 *mk = reinterpret_cast<const uint2 *>(buf);
     const uint32_t *bs = buf + 18 * (size_t)n;
+    // one row loaded ahead of the one handed out
+    uint2 pw = make_uint2(0u, 0u);
+    uint32_t pb = 0;
+    auto load_ahead = [&]() {
+        pw = make_uint2(0u, 0u);
+        if (rowbits != 0) {
+            const uint32_t r = (uint32_t)__builtin_ctz(rowbits);
+            rowbits &= rowbits - 1;
+            pw = mk[(size_t)r * n + k];
+            pb = bs[(size_t)r * n + k];
+        }
+    };
+    load_ahead();
     take_masks<NB, 0>(xps, nullptr, k, pi,
-                          [&](uint64_t &mask, uint32_t &base) {
-                              if (rowbits != 0) {
-                                  const uint32_t r = (uint32_t)__builtin_ctz(rowbits);
-                                  rowbits &= rowbits - 1;
-                                  const uint2 w = mk[(size_t)r * n + k];
-                                  mask = ((uint64_t)w.y << 32) | w.x;
-                                  base = bs[(size_t)r * n + k];
-                              }
-                          },
-                          body);
+                      [&](uint64_t &mask, uint32_t &base) {
+                          mask = ((uint64_t)pw.y << 32) | pw.x;
+                          base = pb;
+                          load_ahead();
+                      },
+                      body);
 }
 
 template <bool GRID, bool SKIP_SELF, bool AUX, bool MASKS = false, bool DEFER = false,
