@@ -778,7 +778,11 @@ int Engine::autotune(float dt, int updates, double ms_out[3], std::string *err)
     double walk_ms[3] = {1e30, 1e30, 1e30}, row_ms[3] = {1e30, 1e30, 1e30};
     GatherVariant walk_v[3] = {keep.density, keep.pressure, keep.viscosity}, row_v[3];
     int rc = check(save(), "autotune snapshot", err);
-    for (size_t ci = 0; ci < cands.size() && !rc; ++ci) {
+    // two passes over the candidates, each kernel keeping its fastest run:
+    // the block sizes of one form differ by about as much as one run's noise
+    const size_t trials = 2 * cands.size();
+    for (size_t ti = 0; ti < trials && !rc; ++ti) {
+        const size_t ci = ti % cands.size();
         LaunchCfg c = keep;
         c.density = c.pressure = c.viscosity = cands[ci].v;
         c.nbr = cands[ci].nbr;

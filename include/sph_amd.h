@@ -180,8 +180,9 @@ int sph_profile_end(sph_handle h, double mean_ms[6], uint32_t *updates);
 /* Autotuner (no reference counterpart): times every variant of the three
  * gather kernels (work order on/off under the reference key, LDS neighbour
  * list on/off, workgroup size 64/128/256, and the row mode) on the current
- * state with `updates` Update(dt) calls per candidate, restoring positions and
- * velocities after each, and keeps the fastest variant per kernel.  All
+ * state with `updates` Update(dt) calls per candidate, twice over the list of
+ * candidates, restoring positions and velocities after each, and keeps the
+ * fastest variant per kernel (its fastest run).  All
  * variants give bit-identical results.  ms (may be NULL) gets the chosen
  * variants' mean ms (density, pressure, viscosity).  Densities and debug
  * tables are recomputed by the next update.  Without it a handle runs the
