@@ -374,6 +374,11 @@ void Engine::invalidate_graph()
         graph_exec_[i] = nullptr;
         graph_[i] = nullptr;
     }
+    if (run_exec_) (void)hipGraphExecDestroy(run_exec_);
+    if (run_graph_) (void)hipGraphDestroy(run_graph_);
+    run_exec_ = nullptr;
+    run_graph_ = nullptr;
+    run_len_ = 0;
 }
 
 int Engine::push_descs(std::string *err)
@@ -860,7 +865,12 @@ int Engine::profile_begin(uint32_t max_updates, std::string *err)
         return SPH_ERR_INVALID;
     }
     prof_ev_.assign(6 * (size_t)max_updates, nullptr);
-    for (hipEvent_t &e : prof_ev_) CK(hipEventCreate(&e), "profile event");
+    // Timing-only events: no system-scope fence (cache writeback and
+    // invalidate) when each is recorded, so the stage boundaries cost the
+    // profiled updates no more than the kernel boundaries of a plain update.
+    // profile_end synchronizes the stream before it reads them.
+    for (hipEvent_t &e : prof_ev_)
+        CK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence), "profile event");
     prof_used_ = 0;
     prof_active_ = true;
     return SPH_OK;
@@ -903,10 +913,44 @@ int Engine::step(float dt, int substeps, bool sync_after, std::string *err)
     }
     const float sub = dt / (float)substeps;
     for (int i = 0; i < substeps; ++i) {
+        const int left = substeps - i;
+        // The remaining updates are steady (this dt, buckets recorded, no
+        // work-order rebuild, no timers): one graph launch runs them all,
+        // which saves the gaps between separate graph launches.
+        if (left > 1 && use_graph_ && !timing_ && !prof_active_ && recorded_ && sub == dt_cached_ &&
+            (grid_ || !cfg_.needs_work())) {
+            const int rc = run_steady(left, err);
+            if (rc) return rc;
+            break;
+        }
         int rc = update(sub, err);
         if (rc) return rc;
     }
     return sync_after ? sync(err) : SPH_OK;
+}
+
+int Engine::run_steady(int n, std::string *err)
+{
+    if (!run_exec_ || run_len_ != n) {
+        if (run_exec_) (void)hipGraphExecDestroy(run_exec_);
+        if (run_graph_) (void)hipGraphDestroy(run_graph_);
+        run_exec_ = nullptr;
+        run_graph_ = nullptr;
+        run_len_ = 0;
+        CK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture");
+        hipError_t le = hipSuccess;
+        for (int u = 0; u < n && le == hipSuccess; ++u) le = enqueue_update(nullptr, false, false);
+        hipError_t ce = hipStreamEndCapture(stream_, &run_graph_);
+        CK(le, "capture updates");
+        CK(ce, "end capture");
+        CK(hipGraphInstantiate(&run_exec_, run_graph_, nullptr, nullptr, 0), "instantiate graph");
+        run_len_ = n;
+    }
+    CK(hipGraphLaunch(run_exec_, stream_), "graph launch");
+    work_updates_ += (uint64_t)n;
+    have_tables_ = true;
+    recorded_ = true;
+    return SPH_OK;
 }
 
 int Engine::sync(std::string *err)

@@ -135,6 +135,7 @@ class Engine {
     hipError_t normalize_scene(uint32_t s);
     // Render positions of scene s from its current state (if they exist).
     hipError_t refresh_render(uint32_t s);
+    int run_steady(int n, std::string *err);  // n steady updates as one graph launch
     bool rebuild_due() const
     {
         return !grid_ && cfg_.needs_work() && (!work_valid_ || work_updates_ % work_period_ == 0);
@@ -158,6 +159,11 @@ class Engine {
     // [rebuild work order? + 2 * fresh predict?]
     hipGraph_t graph_[4] = {nullptr, nullptr, nullptr, nullptr};
     hipGraphExec_t graph_exec_[4] = {nullptr, nullptr, nullptr, nullptr};
+    // run_len_ consecutive steady updates (buckets recorded, no work-order
+    // rebuild) captured as one graph, for the substeps of a step
+    hipGraph_t run_graph_ = nullptr;
+    hipGraphExec_t run_exec_ = nullptr;
+    int run_len_ = 0;
     // The last update's viscosity pass recorded the next update's buckets
     // (keys, tickets, counts) and nothing has changed since: state, settings,
     // layout and dt.  Otherwise the next update starts with predict.
