@@ -656,15 +656,16 @@ int Engine::generate(uint32_t s, const GenPlan &g, std::string *err)
     return SPH_OK;
 }
 
-hipError_t Engine::enqueue_update(hipEvent_t *ev, bool rebuild_work, bool fresh)
+hipError_t Engine::enqueue_update(hipEvent_t *ev, bool rebuild_work, bool fresh, bool mark_predict)
 {
     // ev (optional): 6 events bracketing predict | spatial (+ work order) |
-    // density | pressure | viscosity+integrate.  fresh: the previous update
-    // did not record this one's buckets (its viscosity pass does otherwise)
+    // density | pressure | viscosity+integrate; ev[0] only when mark_predict.
+    // fresh: the previous update did not record this one's buckets (its
+    // viscosity pass does otherwise)
     hipError_t e;
 #define LK(x)                              \
     if ((e = (x)) != hipSuccess) return e;
-    if (ev) LK(hipEventRecord(ev[0], stream_));
+    if (ev && mark_predict) LK(hipEventRecord(ev[0], stream_));
     if (fresh) LK(launch_predict(d_, cfg_, stream_));
     if (ev) LK(hipEventRecord(ev[1], stream_));
     if (rebuild_work) {
@@ -697,7 +698,9 @@ int Engine::update(float dt, std::string *err)
     const bool fresh = !recorded_;
     recorded_ = false;  // until this update is enqueued
     if (prof_active_ && prof_used_ * 6 < prof_ev_.size()) {
-        CK(enqueue_update(&prof_ev_[6 * prof_used_], rebuild, fresh), "launch profiled update");
+        // without predict the first event is left out (one packet fewer)
+        prof_fresh_[prof_used_] = fresh ? 1 : 0;
+        CK(enqueue_update(&prof_ev_[6 * prof_used_], rebuild, fresh, fresh), "launch profiled update");
         ++prof_used_;
     } else if (timing_) {
         CK(enqueue_update(ev_, rebuild, fresh), "launch update");
@@ -865,6 +868,7 @@ int Engine::profile_begin(uint32_t max_updates, std::string *err)
         return SPH_ERR_INVALID;
     }
     prof_ev_.assign(6 * (size_t)max_updates, nullptr);
+    prof_fresh_.assign(max_updates, 0);
     // Timing-only events: no system-scope fence (cache writeback and
     // invalidate) when each is recorded, so the stage boundaries cost the
     // profiled updates no more than the kernel boundaries of a plain update.
@@ -888,6 +892,7 @@ int Engine::profile_end(double mean_ms[6], uint32_t *updates, std::string *err)
     for (uint32_t u = 0; u < prof_used_ && !rc; ++u)
         for (int i = 0; i < 5; ++i) {
             float ms = 0.0f;
+            if (i == 0 && !prof_fresh_[u]) continue;  // no predict: 0 ms
             if (hipEventElapsedTime(&ms, prof_ev_[6 * u + i], prof_ev_[6 * u + i + 1]) != hipSuccess) {
                 rc = SPH_ERR_HIP;
                 break;

@@ -130,7 +130,7 @@ class Engine {
     int alloc(uint32_t capacity, uint32_t table_capacity, std::string *err);
     int push_descs(std::string *err);
     int set_dt(float dt, std::string *err);
-    hipError_t enqueue_update(hipEvent_t *ev, bool rebuild_work, bool fresh);
+    hipError_t enqueue_update(hipEvent_t *ev, bool rebuild_work, bool fresh, bool mark_predict = true);
     // Resident slots of scene s into particle order (ids[k] = p_off + k).
     hipError_t normalize_scene(uint32_t s);
     // Render positions of scene s from its current state (if they exist).
@@ -182,6 +182,7 @@ class Engine {
     float4 *tmp_vel_ = nullptr;
     int staging(std::string *err);
     std::vector<hipEvent_t> prof_ev_;
+    std::vector<uint8_t> prof_fresh_;  // per profiled update: predict ran (its events recorded)
     uint32_t prof_used_ = 0;
     bool prof_active_ = false;
 };
