@@ -275,7 +275,9 @@ int Engine::create(const std::vector<SceneSpec> &scenes, uint32_t capacity,
     // resident order: particle order
     if (!rc) rc = e->check(launch_iota(e->d_.ids, capacity, 0, e->stream_), "resident ids", err);
     if (!rc) rc = e->check(hipStreamSynchronize(e->stream_), "sync", err);
-    for (int i = 0; i < 7 && !rc; ++i) rc = e->check(hipEventCreate(&e->ev_[i]), "event", err);
+    // stage timers: timing only (no system-scope fence when recorded)
+    for (int i = 0; i < 7 && !rc; ++i)
+        rc = e->check(hipEventCreateWithFlags(&e->ev_[i], hipEventDisableSystemFence), "event", err);
     if (rc) {
         delete e;
         return rc;

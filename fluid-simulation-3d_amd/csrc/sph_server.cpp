@@ -244,8 +244,9 @@ int sph_server_create(const sph_server_config *cfg, sph_server *out)
     sph_server_t *s = new sph_server_t();
     s->cfg = *cfg;
     for (int i = 0; i < sph_server_t::kRing; ++i)
-        if (hipEventCreate(&s->ev_start[i]) != hipSuccess ||
-            hipEventCreate(&s->ev_end[i]) != hipSuccess) {
+        // tick timers and in-flight throttling only: no system-scope fence
+        if (hipEventCreateWithFlags(&s->ev_start[i], hipEventDisableSystemFence) != hipSuccess ||
+            hipEventCreateWithFlags(&s->ev_end[i], hipEventDisableSystemFence) != hipSuccess) {
             sph_server_destroy(s);
             return fail(SPH_ERR_HIP, "event creation failed");
         }
