@@ -311,8 +311,9 @@ def main():
                     help="also run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(every box has one MI355X; sharding is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: "
+              "the path runs on a single MI355X here (scenes or slabs would shard across GPUs)",
+              file=sys.stderr)
         return 2
     if S.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
