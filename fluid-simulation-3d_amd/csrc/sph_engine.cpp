@@ -922,10 +922,11 @@ int Engine::step(float dt, int substeps, bool sync_after, std::string *err)
     for (int i = 0; i < substeps; ++i) {
         const int left = substeps - i;
         // The remaining updates are steady (this dt, buckets recorded, no
-        // work-order rebuild, no timers): one graph launch runs them all,
-        // which saves the gaps between separate graph launches.
-        if (left > 1 && use_graph_ && !timing_ && !prof_active_ && recorded_ && sub == dt_cached_ &&
-            (grid_ || !cfg_.needs_work())) {
+        // work-order rebuild, no stage timers): one graph launch runs them
+        // all, which saves the gaps between separate graph launches.  While
+        // profiling, the first update of the step is the profiled one.
+        if (left > 1 && use_graph_ && !timing_ && (!prof_active_ || i > 0) && recorded_ &&
+            sub == dt_cached_ && (grid_ || !cfg_.needs_work())) {
             const int rc = run_steady(left, err);
             if (rc) return rc;
             break;

@@ -167,7 +167,9 @@ int sph_download(sph_handle h, float *pos4, float *vel4, float *dens2);
 int sph_set_timing(sph_handle h, int on);
 int sph_stage_ms(sph_handle h, double ms[6]);
 /* Per-stage device timing of the next `max_updates` updates (up to 2^20)
- * without synchronising: events bracket each stage on the handle's stream.
+ * without synchronising: timing-only events bracket each stage on the
+ * handle's stream.  Within a step of several substeps (sph_step) only the
+ * first update is profiled; the others run unprofiled as one graph launch.
  * sph_profile_end waits, returns the mean ms per update of each stage (order
  * of sph_stage_ms; the kernels are: 0 predict+hash (only when the previous
  * update did not record this one's buckets, else empty), 1 count scan +
