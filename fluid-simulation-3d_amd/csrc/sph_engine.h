@@ -112,9 +112,9 @@ class Engine {
     }
     bool grid() const { return grid_; }
     // Times every gather-kernel variant on the current state (`updates`
-    // Update(dt) calls per candidate, from a snapshot of positions,
-    // velocities and their order that is restored afterwards) and keeps the fastest variant of
-    // each kernel.  ms_out (may be null) gets the chosen variants' mean ms of
+    // Update(dt) calls per candidate, twice over the candidates, from a
+    // snapshot of positions, velocities and their order that is restored
+    // afterwards) and keeps the fastest variant of each kernel.  ms_out (may be null) gets the chosen variants' mean ms of
     // density, pressure, viscosity.  Densities and the debug tables are
     // recomputed by the next update.
     int autotune(float dt, int updates, double ms_out[3], std::string *err);
