@@ -4,7 +4,7 @@
 // counter-based generator (splitmix64).  The per-particle function compiles
 // for host and device from this one source, so the states generated on the
 // GPU (sph_initialize, the server's generated requests) and on the host
-// (sph_generate, used by the checkers) are the same bytes.  Per-scene sizes
+// (sph_generate, used by the oracle's tests) are the same bytes.  Per-scene sizes
 // (lattice dimensions) are planned once on the host.
 #pragma once
 #include <stdint.h>

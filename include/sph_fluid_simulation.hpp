@@ -190,7 +190,7 @@ namespace sph_amd {
 
 // The FluidSimBase backend slot (fluidSimBase.h:5-13) without its render
 // half: a FluidSimHIP for createSimulation (gameApp.cc:87-92) derives from the
-// application's FluidSimBase and forwards to these (see INTEGRATION.md).
+// application's FluidSimBase and forwards to these.
 class FluidSimHIPCore {
   public:
     using vec4 = Physics::Fluid::FluidSimulation::vec4;

@@ -2,8 +2,8 @@
  * oracle.c -- CPU restatement of the reference SPH step (see oracle.h).
  *
  * TEST INFRASTRUCTURE: only tests/, __graft_entry__.smoke() and bench.py's
- * cpu_baseline leg load this (as liboracle.so, through ctypes).  It is the
- * checker, never the thing measured as the product or shipped.
+ * cpu_baseline leg load this (as liboracle.so, through ctypes).  It only
+ * checks results; it is never the thing measured as the product or shipped.
  *
  * Build: oracle/Makefile (gcc -O2 -ffp-contract=off -fno-fast-math -fopenmp).
  * FP32 functions keep the reference's operation order; no FMA contraction.

@@ -1,6 +1,6 @@
 /*
- * oracle.h -- CPU restatement of the reference SPH step, used ONLY as a
- * checker by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
+ * oracle.h -- CPU restatement of the reference SPH step, used ONLY to check
+ * results, by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
  *
  * TEST INFRASTRUCTURE, NOT PRODUCT CODE.  Nothing in the product library
  * (fluid-simulation-3d_amd/) links, loads or calls this file.

@@ -1,7 +1,7 @@
 """ctypes wrapper of liboracle.so -- the CPU restatement of the reference SPH step.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, always as the checker.  The product library
+cpu_baseline leg of bench.py, always to check results.  The product library
 (fluid-simulation-3d_amd/) never imports it.
 
 Each wrapper names the reference function it restates (Allkams/Fluid-Simulation-3D,
