@@ -632,7 +632,10 @@ __device__ __forceinline__ void walk_hash(const SceneDesc &S, const float4 *__re
 // that range and it fits kStageCap records, the wave copies it into its LDS
 // buffer with coalesced loads and the lanes walk their runs from LDS.  The
 // check is exact per lane, so the range assumption never affects a result;
-// rows that fail it are walked from global memory.
+// rows that fail it are walked from global memory.  The buffer holds the
+// records as four planes of kStageBuf floats (x, y, z, w): lanes reading
+// nearby records then hit distinct banks, where 16-byte records at a 16-byte
+// stride made lanes eight records apart collide.
 constexpr int KBG = 4;                          // records per batch of the grid walk
 constexpr uint32_t kStageCap = 120;             // staged records per wave and row
 constexpr uint32_t kStageBuf = kStageCap + KBG; // + padding: a batch may read KBG-1 past the end
@@ -644,9 +647,9 @@ constexpr uint32_t kStageBuf = kStageCap + KBG; // + padding: a batch may read K
 // h, so its cell is one of the 27 around the particle, whose float hashes are
 // pairwise distinct; matching g_q therefore means it is a particle of cell q,
 // which sits in bucket q.  The tests are combined without short-circuit
-// branches.  STAGED: the records are read from the wave's LDS copy
-// src[slot - base] of a range containing [b, e) (the padded buffer makes the
-// reads past e harmless; they never pass); else from xps in global memory.
+// branches.  STAGED: the records are read from the wave's LDS planes at
+// slot - base, for a staged range containing [b, e) (the padded planes make
+// the reads past e harmless; they never pass); else from xps in global memory.
 template <bool STAGED, bool SKIP_SELF, bool AUX, bool MASKS, bool HASH, typename Body>
 __device__ __forceinline__ void walk_run(const float4 *src, uint32_t base, const float4 *__restrict__ xps,
                                          const float4 *__restrict__ aux, uint32_t k,
@@ -659,7 +662,9 @@ __device__ __forceinline__ void walk_run(const float4 *src, uint32_t base, const
 #pragma unroll
         for (int j = 0; j < KBG; ++j) {
             if (STAGED) {
-                rr[j] = src[m + j - base];
+                const float *sp = reinterpret_cast<const float *>(src);
+                const uint32_t q = m + j - base;
+                rr[j] = make_float4(sp[q], sp[kStageBuf + q], sp[2 * kStageBuf + q], sp[3 * kStageBuf + q]);
             } else {
                 rr[j] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(0x7fc00000));  // NaN: no match
                 if (m + j < e) rr[j] = xps[m + j];
@@ -693,21 +698,22 @@ __device__ __forceinline__ void walk_run(const float4 *src, uint32_t base, const
     }
 }
 
-// The mask of a staged run of len <= 64 records src[0, len) when no float-hash
-// test is needed: bit o set when record o lies within h (d2 < thr, the walk's
-// expression).  Every lane starts at offset 0 and steps by KBG, so the offset
+// The mask of a staged run of len <= 64 records (planes x, y, z at sp[0, len),
+// kStageBuf floats apart) when no float-hash test is needed: bit o set when
+// record o lies within h (d2 < thr, the walk's expression).  Every lane starts at offset 0 and steps by KBG, so the offset
 // is one scalar for the wave: a batch's four results form a nibble that one
 // shift by that scalar places, with no per-record branch or bound check.
 // Bits at and past len (a batch reads up to KBG-1 records past the run, inside
 // the padded buffer) are cleared at the end.
-__device__ __forceinline__ uint64_t walk_bits(const float4 *src, uint32_t len, const float4 &pi, float thr)
+__device__ __forceinline__ uint64_t walk_bits(const float *sp, uint32_t len, const float4 &pi, float thr)
 {
     uint64_t mask = 0;
 #pragma unroll 1
     for (uint32_t o = 0; o < len; o += KBG) {
         float4 rr[KBG];
 #pragma unroll
-        for (int j = 0; j < KBG; ++j) rr[j] = src[o + j];
+        for (int j = 0; j < KBG; ++j)
+            rr[j] = make_float4(sp[o + j], sp[kStageBuf + o + j], sp[2 * kStageBuf + o + j], 0.0f);
         uint32_t nib = 0;
 #pragma unroll
         for (int j = 0; j < KBG; ++j) {
@@ -821,8 +827,17 @@ __device__ __forceinline__ uint32_t walk_grid(const SceneDesc &S, const float4 *
                         const uint32_t u = t + cnt < len ? t + cnt : t;
                         const float4 a0 = xps[lo + t];
                         const float4 a1 = xps[lo + u];
-                        stage[t] = a0;
-                        stage[u] = a1;
+                        float *sp = reinterpret_cast<float *>(stage);
+                        sp[t] = a0.x;
+                        sp[kStageBuf + t] = a0.y;
+                        sp[2 * kStageBuf + t] = a0.z;
+                        sp[u] = a1.x;
+                        sp[kStageBuf + u] = a1.y;
+                        sp[2 * kStageBuf + u] = a1.z;
+                        if (!nohash) {  // only the float-hash test reads w
+                            sp[3 * kStageBuf + t] = a0.w;
+                            sp[3 * kStageBuf + u] = a1.w;
+                        }
                     }
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
@@ -837,7 +852,7 @@ __device__ __forceinline__ uint32_t walk_grid(const SceneDesc &S, const float4 *
             };
             if ((DEFER || MASK_ONLY) && !SKIP_SELF && !AUX && staged && nohash &&
                 __ballot(re - rb <= 64u) == __ballot(1))
-                mask = walk_bits(stage + (rb - lo), re - rb, pi, thr);
+                mask = walk_bits(reinterpret_cast<const float *>(stage) + (rb - lo), re - rb, pi, thr);
             else if (staged && nohash)
                 walk_run<true, SKIP_SELF, AUX, MASKS || DEFER, false>(stage, lo, xps, aux, k, pi, thr, rb, re,
                                                                      f0, f1, f2, mask, inline_body);
@@ -855,7 +870,8 @@ __device__ __forceinline__ uint32_t walk_grid(const SceneDesc &S, const float4 *
                     while (mm) {
                         const uint32_t m = rb + (uint32_t)__builtin_ctzll(mm);
                         mm &= mm - 1;
-                        const float4 rec = stage[m - lo];
+                        const float *sp = reinterpret_cast<const float *>(stage);
+                        const float4 rec = make_float4(sp[m - lo], sp[kStageBuf + m - lo], sp[2 * kStageBuf + m - lo], 0.0f);
                         const float ox = rec.x - pi.x, oy = rec.y - pi.y, oz = rec.z - pi.z;
                         body(m, ox, oy, oz, ox * ox + oy * oy + oz * oz, zero);
                     }
