@@ -700,16 +700,21 @@ __device__ __forceinline__ void walk_run(const float4 *src, uint32_t base, const
 
 // The mask of a staged run of len <= 64 records (planes x, y, z at sp[0, len),
 // kStageBuf floats apart) when no float-hash test is needed: bit o set when
-// record o lies within h (d2 < thr, the walk's expression).  Every lane starts at offset 0 and steps by KBG, so the offset
-// is one scalar for the wave: a batch's four results form a nibble that one
-// shift by that scalar places, with no per-record branch or bound check.
-// Bits at and past len (a batch reads up to KBG-1 records past the run, inside
-// the padded buffer) are cleared at the end.
-__device__ __forceinline__ uint64_t walk_bits(const float *sp, uint32_t len, const float4 &pi, float thr)
+// record o lies within h (d2 < thr, the walk's expression).  Every lane starts
+// at offset 0 and steps by KBG, so the offset is one scalar for the wave: a
+// batch's four results form a nibble that one shift by that scalar places,
+// with no per-record branch or bound check.  The two 32-record halves are
+// walked by two loops into 32-bit words (one shift-or per batch instead of a
+// 64-bit shift and two ors).  Bits at and past len (a batch reads up to KBG-1
+// records past the run, inside the padded buffer) are cleared at the end.
+static_assert(32 % KBG == 0, "a batch never straddles the two halves");
+template <uint32_t O0>
+__device__ __forceinline__ uint32_t walk_half(const float *sp, uint32_t len, const float4 &pi, float thr)
 {
-    uint64_t mask = 0;
+    uint32_t bits = 0;
+    const uint32_t end = len < O0 + 32u ? len : O0 + 32u;
 #pragma unroll 1
-    for (uint32_t o = 0; o < len; o += KBG) {
+    for (uint32_t o = O0; o < end; o += KBG) {
         float4 rr[KBG];
 #pragma unroll
         for (int j = 0; j < KBG; ++j)
@@ -721,8 +726,16 @@ __device__ __forceinline__ uint64_t walk_bits(const float *sp, uint32_t len, con
             const float d2 = ox * ox + oy * oy + oz * oz;  // glm::dot order
             nib |= (d2 < thr ? 1u : 0u) << j;
         }
-        mask |= (uint64_t)nib << o;
+        bits |= nib << (o - O0);
     }
+    return bits;
+}
+
+__device__ __forceinline__ uint64_t walk_bits(const float *sp, uint32_t len, const float4 &pi, float thr)
+{
+    const uint32_t lo = walk_half<0>(sp, len, pi, thr);
+    const uint32_t hi = walk_half<32>(sp, len, pi, thr);
+    const uint64_t mask = ((uint64_t)hi << 32) | lo;
     return len >= 64u ? mask : mask & ((1ull << len) - 1ull);
 }
 
