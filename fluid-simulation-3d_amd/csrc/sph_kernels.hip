@@ -502,14 +502,22 @@ __global__ void k_count_reorder(const EngineDesc *__restrict__ ed,
     for (uint32_t t = global_id(); t < n; t += grid_stride()) {
         const uint2 o = order[t];  // (resident slot, particle index)
         if (o.x >= n) continue;   // never taken while the counts start at zero
+        const float4 x0 = pos[o.x], v0 = vel[o.x];  // in flight during the bucket count
         const uint32_t key = keys[o.x];
         const uint32_t lo = bounds[key], hi = min(bounds[key + 1], n);
+        // the bucket's records four at a time, all four loads issued before
+        // any compare (past hi the load repeats record j and is not counted)
         uint32_t before = 0;
-        for (uint32_t j = lo; j < hi; ++j) before += order[j].y < o.y ? 1u : 0u;
+        for (uint32_t j = lo; j < hi; j += 4) {
+            uint32_t y[4];
+#pragma unroll
+            for (uint32_t q = 0; q < 4; ++q) y[q] = order[j + q < hi ? j + q : j].y;
+#pragma unroll
+            for (uint32_t q = 0; q < 4; ++q) before += (j + q < hi && y[q] < o.y) ? 1u : 0u;
+        }
         const uint32_t slot = lo + before;
         if (slot >= hi) continue;  // as above
         const SceneDesc &S = scene_of_slot<MULTI>(sd, scene_of, o.x);
-        const float4 x0 = pos[o.x], v0 = vel[o.x];
         const Predicted P = predict_particle(S, x0, v0);
         sids[slot] = o.y;
         xps[slot] = make_float4(P.px, P.py, P.pz, (float)P.hash);
