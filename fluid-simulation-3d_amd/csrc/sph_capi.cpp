@@ -565,6 +565,15 @@ int sph_batch_set_variants(sph_batch b, const int32_t v[13])
     return set_variants(b->eng, v);
 }
 
+int sph_debug_set_scan_spin(sph_handle h, uint32_t sleeps)
+{
+    SPH_GUARD_BEGIN
+    SPH_NEED(h);
+    std::string err;
+    return done(h->eng->set_scan_spin(sleeps, &err), err);
+    SPH_GUARD_END
+}
+
 int sph_count_nonfinite(sph_handle h, uint64_t *count)
 {
     SPH_GUARD_BEGIN
@@ -594,6 +603,15 @@ int sph_debug_predicted(sph_handle h, float *xp4)
     if (!xp4) return fail(SPH_ERR_INVALID, "null output");
     std::string err;
     return done(h->eng->debug_predicted(0, xp4, &err), err);
+    SPH_GUARD_END
+}
+
+int sph_debug_velocities(sph_handle h, float *v_ext4, float *v_press4)
+{
+    SPH_GUARD_BEGIN
+    SPH_NEED(h);
+    std::string err;
+    return done(h->eng->debug_velocities(0, v_ext4, v_press4, &err), err);
     SPH_GUARD_END
 }
 

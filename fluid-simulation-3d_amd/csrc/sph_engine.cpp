@@ -249,6 +249,8 @@ int Engine::create(const std::vector<SceneSpec> &scenes, uint32_t capacity,
         memcpy(e->ed_h_->stops, stops, sizeof(stops));
     }
     work_key_layout(e->ed_h_->n_scenes, e->ed_h_);
+    e->ed_h_->scan_spin = kScanSpinDefault;
+    e->invalid_.assign(scenes.size(), 0);
     e->d_.multi = multi || scenes.size() > 1;
     e->grid_ = grid;
     e->d_.grid = grid;
@@ -336,6 +338,8 @@ int Engine::alloc(uint32_t cap, uint32_t tcap, std::string *err)
     if (!rc) rc = dm((void **)&d_.nbr, u4 * 64, "neighbour rows");
     if (!rc) rc = dm((void **)&d_.ncnt, u4, "neighbour counts");
     if (!rc) rc = dm((void **)&d_.sflags, sizeof(uint32_t) * scene_cap_, "scene flags");
+    if (!rc) rc = dm((void **)&d_.scan_err, sizeof(uint32_t), "scan error flag");
+    if (!rc) rc = check(hipMemset(d_.scan_err, 0, sizeof(uint32_t)), "scan error flag", err);
     // wrapped until a predict pass says otherwise
     if (!rc) rc = check(hipMemset(d_.sflags, 0xff, sizeof(uint32_t) * scene_cap_), "scene flags", err);
     d_.ed = d_ed_;
@@ -354,7 +358,7 @@ Engine::~Engine()
                     d_.ids,      d_.ps,       d_.order,    d_.sids,
                     d_.hist,     d_.bounds,   d_.count,    d_.chain,
                     d_.wkeys[0], d_.wkeys[1], d_.wvals[0], d_.wvals[1], d_.rank,
-                    d_.nbr,      d_.ncnt,     d_.sflags};
+                    d_.nbr,      d_.ncnt,     d_.sflags,   d_.scan_err};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (tmp_pos_) (void)hipFree(tmp_pos_);
@@ -485,6 +489,10 @@ int Engine::repack(const std::vector<RepackItem> &items, std::string *err)
                           hipMemcpyHostToDevice, stream_), "repack scene map");
     // host sources (caller buffers, scene_of) must stay alive until the copies end
     CK(hipStreamSynchronize(stream_), "repack sync");
+    std::vector<uint8_t> invalid(items.size(), 0);
+    for (size_t q = 0; q < items.size(); ++q)
+        if (items[q].old_scene >= 0) invalid[q] = invalid_[items[q].old_scene];
+    invalid_.swap(invalid);
     sd_h_.swap(sd);
     params_.swap(params);
     ed_h_->n_total = (uint32_t)n_total;
@@ -602,6 +610,7 @@ int Engine::upload(uint32_t s, const float *pos4, const float *vel4, std::string
            "upload velocities");
     CK(refresh_render(s), "upload render positions");
     CK(hipStreamSynchronize(stream_), "sync after upload");
+    if (pos4 && vel4) invalid_[s] = 0;  // (a partial upload keeps the rest of a failed state)
     return SPH_OK;
 }
 
@@ -655,6 +664,7 @@ int Engine::generate(uint32_t s, const GenPlan &g, std::string *err)
     CK(launch_iota(d_.ids + d.p_off, d.n, d.p_off, stream_), "generate order");
     CK(refresh_render(s), "generate render positions");
     CK(hipStreamSynchronize(stream_), "sync after generate");
+    invalid_[s] = 0;
     return SPH_OK;
 }
 
@@ -964,7 +974,31 @@ int Engine::run_steady(int n, std::string *err)
 int Engine::sync(std::string *err)
 {
     CK(hipStreamSynchronize(stream_), "sync");
-    return SPH_OK;
+    return device_status(-1, err);
+}
+
+int Engine::device_status(int s, std::string *err)
+{
+    uint32_t flag = 0;
+    CK(hipMemcpy(&flag, d_.scan_err, sizeof(uint32_t), hipMemcpyDeviceToHost), "read device status");
+    if (flag) {
+        for (uint8_t &v : invalid_) v = 1;
+        CK(hipMemset(d_.scan_err, 0, sizeof(uint32_t)), "clear device status");
+    }
+    bool bad = false;
+    if (s >= 0) bad = (size_t)s < invalid_.size() && invalid_[s];
+    else
+        for (uint8_t v : invalid_) bad = bad || v;
+    if (!bad) return SPH_OK;
+    *err = "the count scan's bounded look-back wait gave up during an update (device busy "
+           "beyond its bound): the state is invalid until new state is uploaded or generated";
+    return SPH_ERR_DEVICE;
+}
+
+int Engine::set_scan_spin(uint32_t sleeps, std::string *err)
+{
+    ed_h_->scan_spin = sleeps;
+    return push_descs(err);
 }
 
 int Engine::prime(std::string *err)
@@ -1009,6 +1043,7 @@ int Engine::download(uint32_t s, float *pos4, float *vel4, float *dens2, std::st
     }
     const SceneDesc &d = sd_h_[s];
     CK(hipStreamSynchronize(stream_), "sync before download");
+    if (int rc = device_status((int)s, err)) return rc;
     if (pos4 || vel4) {
         // resident order -> particle order on the host
         std::vector<uint32_t> ids(d.n);
@@ -1063,6 +1098,7 @@ int Engine::debug_spatial(uint32_t s, uint32_t *hash_id, uint32_t *key_id, uint3
     // index of each sorted slot, the exact hash in ps.w, the bucket bounds
     const SceneDesc &d = sd_h_[s];
     CK(hipStreamSynchronize(stream_), "sync");
+    if (int rc = device_status((int)s, err)) return rc;
     std::vector<float4> ps(d.n);
     std::vector<uint32_t> si(d.n), bd((size_t)d.table + 1), sk(d.n, 0xffffffffu);
     CK(hipMemcpy(ps.data(), d_.ps + d.p_off, sizeof(float4) * d.n, hipMemcpyDeviceToHost), "ps");
@@ -1103,6 +1139,7 @@ int Engine::debug_predicted(uint32_t s, float *xp4, std::string *err)
     // the sorted copy of the last update's predicted positions
     const SceneDesc &d = sd_h_[s];
     CK(hipStreamSynchronize(stream_), "sync");
+    if (int rc = device_status((int)s, err)) return rc;
     std::vector<float4> xps(d.n);
     std::vector<uint32_t> si(d.n);
     CK(hipMemcpy(xps.data(), d_.xps + d.p_off, sizeof(float4) * d.n, hipMemcpyDeviceToHost), "xps");
@@ -1115,6 +1152,39 @@ int Engine::debug_predicted(uint32_t s, float *xp4, std::string *err)
         xp4[4 * i + 1] = xps[k].y;
         xp4[4 * i + 2] = xps[k].z;
         xp4[4 * i + 3] = 0.0f;
+    }
+    return SPH_OK;
+}
+
+int Engine::debug_velocities(uint32_t s, float *v_ext4, float *v_press4, std::string *err)
+{
+    if (s >= sd_h_.size()) {
+        *err = "scene index out of range";
+        return SPH_ERR_INVALID;
+    }
+    if (!have_tables_) {
+        *err = "no update has run yet";
+        return SPH_ERR_STATE;
+    }
+    // vs (after the external force) and vs2 (after pressure) of the last
+    // update, in its sorted order
+    const SceneDesc &d = sd_h_[s];
+    CK(hipStreamSynchronize(stream_), "sync");
+    if (int rc = device_status((int)s, err)) return rc;
+    std::vector<uint32_t> si(d.n);
+    std::vector<float4> buf(d.n);
+    CK(hipMemcpy(si.data(), d_.sids + d.p_off, sizeof(uint32_t) * d.n, hipMemcpyDeviceToHost),
+       "sorted idx");
+    for (int a = 0; a < 2; ++a) {
+        float *out = a ? v_press4 : v_ext4;
+        if (!out) continue;
+        CK(hipMemcpy(buf.data(), (a ? d_.vs2 : d_.vs) + d.p_off, sizeof(float4) * d.n,
+                     hipMemcpyDeviceToHost), "stage velocities");
+        for (uint32_t k = 0; k < d.n; ++k) {
+            const uint32_t i = si[k] - d.p_off;
+            if (i >= d.n) continue;
+            memcpy(out + 4 * (size_t)i, &buf[k], sizeof(float4));
+        }
     }
     return SPH_OK;
 }

@@ -84,12 +84,18 @@ class Engine {
     // `substeps` Update(dt / substeps) calls; blocks until done when sync.
     int step(float dt, int substeps, bool sync, std::string *err);
     int sync(std::string *err);
+    // Bound of the count scan's look-back wait in 64-clock sleeps (default
+    // kScanSpinDefault).  A wait that reaches it makes the update's results
+    // invalid, and the next synchronising call reports SPH_ERR_DEVICE.
+    int set_scan_spin(uint32_t sleeps, std::string *err);
+    static constexpr uint32_t kScanSpinDefault = 1u << 16;
     // InitializeData tail (cc:144-145): spatial lookup + densities of the current positions.
     int prime(std::string *err);
     int download(uint32_t s, float *pos4, float *vel4, float *dens2, std::string *err);
     int debug_spatial(uint32_t s, uint32_t *hash_id, uint32_t *key_id, uint32_t *sorted_key,
                       uint32_t *sorted_idx, uint32_t *start, uint32_t *end, std::string *err);
     int debug_predicted(uint32_t s, float *xp4, std::string *err);
+    int debug_velocities(uint32_t s, float *v_ext4, float *v_press4, std::string *err);
     int debug_neighbours(uint32_t s, uint64_t *offs, uint32_t *list, uint64_t cap,
                          uint64_t *total, std::string *err);
     int count_nonfinite(uint64_t *count, std::string *err);
@@ -141,6 +147,13 @@ class Engine {
         return !grid_ && cfg_.needs_work() && (!work_valid_ || work_updates_ % work_period_ == 0);
     }
     int check(hipError_t e, const char *what, std::string *err);
+    // After a stream synchronisation: a device-side failure of the updates
+    // since the last check (the count scan's bounded wait gave up) marks every
+    // scene's state invalid; returns SPH_ERR_DEVICE while scene s (any scene
+    // when s < 0) is invalid.  Uploading or generating a scene's state makes
+    // it valid again.
+    int device_status(int s, std::string *err);
+    std::vector<uint8_t> invalid_;  // per scene: state left by a failed update
 
     hipStream_t stream_ = nullptr;
     DevState d_{};

@@ -374,14 +374,18 @@ __device__ __forceinline__ void load_counts(const uint32_t *__restrict__ count, 
 // words at a time, back to the nearest chunk whose inclusive prefix is
 // published -- and publishes its own inclusive prefix.  A workgroup waits only
 // for chunks taken before its own, by workgroups already running, so every
-// wait ends; the wait is bounded as well (past the bound the result is wrong,
-// never a hang).  chain: [gridDim.x] status words, then the chunk counter; all
-// zero on entry (the reorder clears them).  The workgroup of the last chunk
-// writes bounds[T] and publishes the scene flags.
+// wait ends; the wait is bounded as well (EngineDesc::scan_spin sleeps of 64
+// clocks).  A wait that reaches the bound counts the unpublished chunks as
+// empty and raises *scan_err, which the host reads at its next
+// synchronisation and reports as an error (the update's results are then
+// invalid); every bound written is clamped to the particle count, so the
+// kernels after it stay inside the particle arrays either way.  chain:
+// [gridDim.x] status words, then the chunk counter; all zero on entry (the
+// reorder clears them).  The workgroup of the last chunk writes bounds[T] and
+// publishes the scene flags.
 constexpr uint64_t kChainAgg = 1ull << 62;    // the chunk's count sum
 constexpr uint64_t kChainInc = 2ull << 62;    // the inclusive prefix through the chunk
 constexpr uint64_t kChainFlags = 3ull << 62;
-constexpr uint32_t kChainSpinMax = 1u << 16;  // sleeps of 64 clocks
 
 __device__ __forceinline__ uint64_t chain_load(const uint64_t *p)
 {
@@ -397,11 +401,13 @@ __global__ __launch_bounds__(kSortBlock) void k_count_scan(const EngineDesc *__r
                                                             const uint32_t *__restrict__ count,
                                                             uint64_t *__restrict__ chain,
                                                             uint32_t *__restrict__ bounds,
-                                                            uint32_t *__restrict__ sflags)
+                                                            uint32_t *__restrict__ sflags,
+                                                            uint32_t *__restrict__ scan_err)
 {
     __shared__ uint32_t wsum[kSortWaves];
     __shared__ uint32_t chunk_s, base_s;
     const uint32_t T = ed->table_total, tid = threadIdx.x, nb = gridDim.x;
+    const uint32_t n = ed->n_total, spin_max = ed->scan_spin;
     if (tid == 0) chunk_s = atomicAdd(reinterpret_cast<uint32_t *>(chain + nb), 1u);
     __syncthreads();
     const uint32_t c = chunk_s;
@@ -421,11 +427,13 @@ __global__ __launch_bounds__(kSortBlock) void k_count_scan(const EngineDesc *__r
         for (int32_t j0 = (int32_t)c - 1; j0 >= 0; j0 -= 64) {
             const int32_t j = j0 - (int32_t)lane;  // lane 0: the nearest chunk
             uint64_t st = j >= 0 ? chain_load(&chain[j]) : kChainInc;  // before chunk 0: 0
-            while (__ballot((st & kChainFlags) == 0) != 0 && spins < kChainSpinMax) {
+            while (__ballot((st & kChainFlags) == 0) != 0 && spins < spin_max) {
                 __builtin_amdgcn_s_sleep(1);
                 ++spins;
                 if ((st & kChainFlags) == 0) st = chain_load(&chain[j]);
             }
+            // the bound is reached: the chunks still unpublished count as empty
+            if (__ballot((st & kChainFlags) == 0) != 0 && lane == 0) *scan_err = 1u;
             const uint64_t inc = __ballot((st & kChainFlags) == kChainInc);
             // the chunks up to the nearest inclusive prefix (all 64 if none)
             const uint32_t stop = inc ? (uint32_t)__builtin_ctzll(inc) : 63u;
@@ -441,13 +449,14 @@ __global__ __launch_bounds__(kSortBlock) void k_count_scan(const EngineDesc *__r
         }
     }
     __syncthreads();
+    // (min: an exact prefix never exceeds n; one after a given-up wait stays in range)
     uint32_t p = base_s + local;
     if (b0 + kCountPer <= T) {
         uint4 *q = reinterpret_cast<uint4 *>(bounds + b0);
         uint32_t o[kCountPer];
 #pragma unroll
         for (uint32_t j = 0; j < kCountPer; ++j) {
-            o[j] = p;
+            o[j] = min(p, n);
             p += cnt[j];
         }
         q[0] = make_uint4(o[0], o[1], o[2], o[3]);
@@ -455,12 +464,12 @@ __global__ __launch_bounds__(kSortBlock) void k_count_scan(const EngineDesc *__r
     } else {
 #pragma unroll
         for (uint32_t j = 0; j < kCountPer; ++j) {
-            if (b0 + j < T) bounds[b0 + j] = p;
+            if (b0 + j < T) bounds[b0 + j] = min(p, n);
             p += cnt[j];
         }
     }
     if (c == nb - 1) {
-        if (tid == 0) bounds[T] = base_s + total;
+        if (tid == 0) bounds[T] = min(base_s + total, n);
         for (uint32_t q = tid; q < ed->n_scenes; q += kSortBlock) sflags[q] = count[T + q];
     }
 }
@@ -485,36 +494,95 @@ __global__ void k_count_scatter(const EngineDesc *__restrict__ ed,
 // float(hash), the reference record's .y, cc:480), vs (velocity after the
 // external force), ps (position, w = the exact hash bits) and sids, the
 // particle index of each sorted slot.
+//
+// The count: a work-item whose bucket holds at most kRankDirect records reads
+// them itself (four loads in flight per round).  Records of larger buckets are
+// ranked by the whole workgroup: it walks such a bucket once in tiles of
+// blockDim.x particle indices staged in LDS, every member work-item counting
+// the smaller ones with broadcast LDS reads.  So a bucket of k records costs
+// k global reads per workgroup that holds some of its records, not per
+// record, and the per-record work is k LDS comparisons: a degenerate scene
+// (every particle in one cell, or a tiny reference table) stays bounded where
+// per-record global reads would take k^2 of them.  The workgroup's records
+// are consecutive sorted places, so few large buckets meet in one workgroup.
+// Launched with kReorderBlock work-items per workgroup.
+constexpr uint32_t kRankDirect = 64;
+constexpr uint32_t kReorderBlock = 256;
+
 template <bool MULTI>
-__global__ void k_count_reorder(const EngineDesc *__restrict__ ed,
-                                const SceneDesc *__restrict__ sd,
-                                const uint32_t *__restrict__ scene_of,
-                                const uint32_t *__restrict__ keys,
-                                const uint32_t *__restrict__ bounds,
-                                const uint2 *__restrict__ order, uint32_t *__restrict__ sids,
-                                const float4 *__restrict__ vel, const float4 *__restrict__ pos,
-                                float4 *__restrict__ xps, float4 *__restrict__ vs,
-                                float4 *__restrict__ ps, uint32_t *__restrict__ rank,
-                                uint32_t *__restrict__ count, uint64_t *__restrict__ chain,
-                                uint32_t chain_words)
+__global__ __launch_bounds__(kReorderBlock) void k_count_reorder(
+    const EngineDesc *__restrict__ ed, const SceneDesc *__restrict__ sd,
+    const uint32_t *__restrict__ scene_of, const uint32_t *__restrict__ keys,
+    const uint32_t *__restrict__ bounds, const uint2 *__restrict__ order,
+    uint32_t *__restrict__ sids, const float4 *__restrict__ vel, const float4 *__restrict__ pos,
+    float4 *__restrict__ xps, float4 *__restrict__ vs, float4 *__restrict__ ps,
+    uint32_t *__restrict__ rank, uint32_t *__restrict__ count, uint64_t *__restrict__ chain,
+    uint32_t chain_words)
 {
-    const uint32_t n = ed->n_total;
-    for (uint32_t t = global_id(); t < n; t += grid_stride()) {
-        const uint2 o = order[t];  // (resident slot, particle index)
-        if (o.x >= n) continue;   // never taken while the counts start at zero
-        const float4 x0 = pos[o.x], v0 = vel[o.x];  // in flight during the bucket count
-        const uint32_t key = keys[o.x];
-        const uint32_t lo = bounds[key], hi = min(bounds[key + 1], n);
-        // the bucket's records four at a time, all four loads issued before
-        // any compare (past hi the load repeats record j and is not counted)
-        uint32_t before = 0;
-        for (uint32_t j = lo; j < hi; j += 4) {
-            uint32_t y[4];
-#pragma unroll
-            for (uint32_t q = 0; q < 4; ++q) y[q] = order[j + q < hi ? j + q : j].y;
-#pragma unroll
-            for (uint32_t q = 0; q < 4; ++q) before += (j + q < hi && y[q] < o.y) ? 1u : 0u;
+    __shared__ uint32_t tile[kReorderBlock];
+    __shared__ uint32_t next_key;
+    const uint32_t n = ed->n_total, B = blockDim.x, tid = threadIdx.x;
+    // whole workgroups iterate together (the cooperative rank synchronises)
+    for (uint32_t t0 = blockIdx.x * B; t0 < n; t0 += gridDim.x * B) {
+        const uint32_t t = t0 + tid;
+        uint2 o = make_uint2(n, 0u);  // (resident slot, particle index)
+        if (t < n) o = order[t];
+        const bool have = o.x < n;    // always, while the counts start at zero
+        float4 x0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), v0 = x0;
+        uint32_t key = 0, lo = 0, hi = 0;
+        if (have) {
+            x0 = pos[o.x];  // in flight during the bucket count
+            v0 = vel[o.x];
+            key = keys[o.x];
+            lo = bounds[key];
+            hi = min(bounds[key + 1], n);
         }
+        const bool big = have && hi > lo && hi - lo > kRankDirect;
+        uint32_t before = 0;
+        if (have && !big) {
+            // the bucket's records four at a time, all four loads issued
+            // before any compare (past hi the load repeats record j and is not
+            // counted)
+            for (uint32_t j = lo; j < hi; j += 4) {
+                uint32_t y[4];
+#pragma unroll
+                for (uint32_t q = 0; q < 4; ++q) y[q] = order[j + q < hi ? j + q : j].y;
+#pragma unroll
+                for (uint32_t q = 0; q < 4; ++q) before += (j + q < hi && y[q] < o.y) ? 1u : 0u;
+            }
+        }
+        if (__syncthreads_or(big)) {
+            bool pending = big;
+            for (;;) {
+                // the next large bucket of this workgroup (the smallest key left)
+                if (tid == 0) next_key = 0xffffffffu;
+                __syncthreads();
+                if (pending) atomicMin(&next_key, key);
+                __syncthreads();
+                const uint32_t bk = next_key;
+                if (bk == 0xffffffffu) break;
+                const uint32_t blo = bounds[bk], bhi = min(bounds[bk + 1], n);
+                const bool mine = pending && key == bk;
+                for (uint32_t j0 = blo; j0 < bhi; j0 += B) {
+                    const uint32_t m = min(B, bhi - j0);
+                    __syncthreads();  // the previous tile is consumed
+                    if (tid < m) tile[tid] = order[j0 + tid].y;
+                    __syncthreads();
+                    if (mine) {
+                        uint32_t c0 = 0, c1 = 0;
+                        uint32_t q = 0;
+                        for (; q + 1 < m; q += 2) {
+                            c0 += tile[q] < o.y ? 1u : 0u;
+                            c1 += tile[q + 1] < o.y ? 1u : 0u;
+                        }
+                        if (q < m) c0 += tile[q] < o.y ? 1u : 0u;
+                        before += c0 + c1;
+                    }
+                }
+                pending = pending && !mine;
+            }
+        }
+        if (!have) continue;
         const uint32_t slot = lo + before;
         if (slot >= hi) continue;  // as above
         const SceneDesc &S = scene_of_slot<MULTI>(sd, scene_of, o.x);
@@ -1704,17 +1772,19 @@ hipError_t launch_radix_sort(const DevState &s, uint32_t *const keys[2], uint32_
 hipError_t launch_bucket_sort(const DevState &s, const LaunchCfg &c, hipStream_t st)
 {
     const uint32_t nb = s.count_blocks, g = grid_for(s.capacity, c.block_stream);
-    k_count_scan<<<nb, kSortBlock, 0, st>>>(s.ed, s.count, s.chain, s.bounds, s.sflags);
+    k_count_scan<<<nb, kSortBlock, 0, st>>>(s.ed, s.count, s.chain, s.bounds, s.sflags,
+                                             s.scan_err);
     k_count_scatter<<<g, c.block_stream, 0, st>>>(s.ed, s.keys[0], s.keys[1], s.ids, s.bounds,
                                                    s.order);
     uint32_t *const rk = s.grid ? nullptr : s.rank;
+    const uint32_t gr = grid_for(s.capacity, kReorderBlock);
     if (s.multi)
-        k_count_reorder<true><<<g, c.block_stream, 0, st>>>(s.ed, s.sd, s.scene_of, s.keys[0],
+        k_count_reorder<true><<<gr, kReorderBlock, 0, st>>>(s.ed, s.sd, s.scene_of, s.keys[0],
                                                              s.bounds, s.order, s.sids, s.vel,
                                                              s.pos, s.xps, s.vs, s.ps, rk,
                                                              s.count, s.chain, nb + 1);
     else
-        k_count_reorder<false><<<g, c.block_stream, 0, st>>>(s.ed, s.sd, s.scene_of, s.keys[0],
+        k_count_reorder<false><<<gr, kReorderBlock, 0, st>>>(s.ed, s.sd, s.scene_of, s.keys[0],
                                                               s.bounds, s.order, s.sids, s.vel,
                                                               s.pos, s.xps, s.vs, s.ps, rk,
                                                               s.count, s.chain, nb + 1);
@@ -1828,6 +1898,12 @@ hipError_t launch_unpermute_state(const DevState &s, uint32_t off, uint32_t n, f
 hipError_t launch_iota(uint32_t *a, uint32_t n, uint32_t first, hipStream_t st)
 {
     k_iota<<<grid_for(n, 256), 256, 0, st>>>(a, n, first);
+    SPH_LAUNCH_CHECK();
+}
+
+hipError_t launch_clear_words(uint32_t *a, uint32_t n, hipStream_t st)
+{
+    k_count_clear<<<grid_for(n, 256), 256, 0, st>>>(a, n);
     SPH_LAUNCH_CHECK();
 }
 

@@ -56,6 +56,7 @@ struct DevState {
                          // (copied from the count flags by the bucket sort)
     uint32_t *nbr;       // [capacity * 64] neighbour rows by work index (nbr mode)
     uint32_t *ncnt;      // [capacity] neighbour count per row
+    uint32_t *scan_err;  // [1] raised when the count scan's bounded wait gave up
     uint32_t capacity;       // particle slots allocated
     uint32_t table_capacity; // bucket slots allocated
 };
@@ -128,6 +129,8 @@ hipError_t launch_unpermute_state(const DevState &s, uint32_t off, uint32_t n, f
                                   float4 *dst_vel, hipStream_t st);
 // a[k] = first + k for k < n.
 hipError_t launch_iota(uint32_t *a, uint32_t n, uint32_t first, hipStream_t st);
+// a[k] = 0 for k < n, by a fill kernel (no memset node when captured).
+hipError_t launch_clear_words(uint32_t *a, uint32_t n, hipStream_t st);
 // Neighbour lists for parity checks (particle order, walk order, self included).
 hipError_t launch_neighbour_count(const DevState &s, uint32_t *counts, hipStream_t st);
 hipError_t launch_neighbour_fill(const DevState &s, const uint64_t *offs,

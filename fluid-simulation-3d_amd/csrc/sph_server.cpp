@@ -88,7 +88,7 @@ struct sph_server_t {
     std::vector<double> wait_ms;        // submit -> admit
     uint64_t particle_updates = 0;      // particles x updates advanced
     double busy_s = 0;                  // wall time spent in ticks with work
-    uint64_t finished = 0, admitted = 0, rejected = 0, cancelled = 0;
+    uint64_t finished = 0, admitted = 0, rejected = 0, cancelled = 0, admission_failures = 0;
     // GPU timing of ticks: event pairs in a ring; at most kInFlight ticks are
     // enqueued ahead of the GPU, so host-side admission overlaps device work
     static constexpr int kRing = 8;
@@ -117,11 +117,13 @@ int ensure_capacity(sph_server_t *s, uint64_t need_n, uint64_t need_t, std::stri
     uint64_t tcap = e ? e->table_capacity() : 2 * cap;
     while (cap < need_n) cap *= 2;
     while (tcap < need_t) tcap *= 2;
-    cap = std::min<uint64_t>(cap, 1ull << 31);
+    const uint64_t cap_max = s->cfg.max_capacity ? s->cfg.max_capacity : (1ull << 31);
+    cap = std::min<uint64_t>(cap, cap_max);
     tcap = std::min<uint64_t>(tcap, 1ull << 31);
     if (cap < need_n || tcap < need_t) {
-        *err = "capacity limit";
-        return SPH_ERR_BUDGET;
+        *err = "growing the device layout beyond its particle or bucket limit (max_capacity " +
+               std::to_string(cap_max) + ")";
+        return SPH_ERR_OOM;
     }
     sph::Engine *ne = nullptr;
     int rc = sph::Engine::create({}, (uint32_t)cap, (uint32_t)tcap, s->cfg.max_running, &ne, err,
@@ -222,6 +224,7 @@ void sph_server_default_config(sph_server_config *c)
     c->dt = 1.0f / 120.0f;
     c->substeps = 3;
     c->updates_per_tick = 1;
+    c->max_capacity = 0;
 }
 
 uint64_t sph_server_request_bytes(const sph_params *p, uint32_t n, uint32_t table_size)
@@ -341,26 +344,49 @@ int sph_server_tick(sph_server s, sph_server_tick_info *info)
     if (!s) return fail(SPH_ERR_INVALID, "null server");
     const double t0 = now_s();
     std::string err;
-    // 1. admit in arrival order while a slot is free and the budget holds
+    // 1. admit in arrival order while a slot is free and the budget holds;
+    // the requests change state only once the device layout holds them (a
+    // failed repack or growth leaves them queued, the budget uncharged)
     std::vector<uint64_t> admit;
     size_t slots = 0;
+    uint64_t charge = 0;
     for (uint64_t id : s->running)
         if (s->reqs.at(id).state == SPH_REQ_RUNNING) ++slots;
-    while (!s->queue.empty() && slots + admit.size() < s->cfg.max_running) {
-        Request &r = s->reqs.at(s->queue.front());
-        if (s->used_bytes + r.bytes > s->cfg.hbm_budget_bytes) break;  // head of line waits
-        s->used_bytes += r.bytes;
-        r.state = SPH_REQ_RUNNING;
-        r.t_admit = now_s();
-        s->wait_ms.push_back((r.t_admit - r.t_submit) * 1e3);
+    for (size_t q = 0; q < s->queue.size() && slots + admit.size() < s->cfg.max_running; ++q) {
+        const Request &r = s->reqs.at(s->queue[q]);
+        if (s->used_bytes + charge + r.bytes > s->cfg.hbm_budget_bytes) break;  // head of line waits
+        charge += r.bytes;
         admit.push_back(r.id);
-        s->queue.pop_front();
-        ++s->admitted;
     }
     if (!admit.empty() || s->dirty) {
         int rc = repack(s, admit, &err);
+        if (rc == SPH_ERR_OOM && !admit.empty()) {
+            // the layout cannot grow to hold them (device memory, or
+            // max_capacity): they wait at the head of the queue, as for the
+            // budget, while the running requests go on
+            ++s->admission_failures;
+            admit.clear();
+            charge = 0;
+            bool any_running = false;
+            for (uint64_t id : s->running)
+                any_running = any_running || s->reqs.at(id).state == SPH_REQ_RUNNING;
+            rc = s->dirty ? repack(s, admit, &err) : SPH_OK;  // departures still apply
+            if (!rc && !any_running)
+                return fail(SPH_ERR_OOM, "the request at the head of the queue cannot be admitted "
+                                         "even with nothing else running: " + err);
+        }
         if (rc) return fail(rc, err);
     }
+    const double t_admit = now_s();
+    for (uint64_t id : admit) {
+        Request &r = s->reqs.at(id);
+        r.state = SPH_REQ_RUNNING;
+        r.t_admit = t_admit;
+        s->wait_ms.push_back((r.t_admit - r.t_submit) * 1e3);
+        s->queue.pop_front();
+        ++s->admitted;
+    }
+    s->used_bytes += charge;
     // 2. advance every running request
     uint32_t finished = 0;
     uint64_t particles = 0;
@@ -462,7 +488,7 @@ int sph_server_status(sph_server s, uint64_t id, int32_t *state, uint64_t *updat
     SRV_GUARD_END
 }
 
-int sph_server_fetch(sph_server s, uint64_t id, float *pos4, float *vel4, float *dens2)
+int sph_server_fetch(sph_server s, uint64_t id, uint32_t n, float *pos4, float *vel4, float *dens2)
 {
     SRV_GUARD_BEGIN
     if (!s) return fail(SPH_ERR_INVALID, "null server");
@@ -471,6 +497,9 @@ int sph_server_fetch(sph_server s, uint64_t id, float *pos4, float *vel4, float 
         return fail(SPH_ERR_NOT_FOUND, "unknown request id");
     Request &r = it->second;
     if (r.state != SPH_REQ_DONE) return fail(SPH_ERR_STATE, "request not finished");
+    if (n != r.n)
+        return fail(SPH_ERR_INVALID, "n must equal the request's particle count (" +
+                                         std::to_string(r.n) + ")");
     if (pos4) memcpy(pos4, r.pos.data(), sizeof(float) * r.pos.size());
     if (vel4) memcpy(vel4, r.vel.data(), sizeof(float) * r.vel.size());
     if (dens2) memcpy(dens2, r.dens.data(), sizeof(float) * r.dens.size());
@@ -550,6 +579,7 @@ int sph_server_get_stats(sph_server s, sph_server_stats *st)
     st->wait_ms_p50 = percentile(s->wait_ms, 50);
     st->wait_ms_p99 = percentile(s->wait_ms, 99);
     st->capacity = s->eng ? s->eng->capacity() : 0;
+    st->admission_failures = s->admission_failures;
     return SPH_OK;
     SRV_GUARD_END
 }

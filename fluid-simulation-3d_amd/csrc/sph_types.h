@@ -65,6 +65,7 @@ struct EngineDesc {
     uint32_t n_scenes;
     uint32_t scene_shift;  // work-order key: scene index << scene_shift (32: one scene)
     uint32_t morton_drop;  // work-order key: Morton code >> morton_drop
-    uint32_t pad[3];
+    uint32_t scan_spin;    // bound of the count scan's look-back wait, in 64-clock sleeps
+    uint32_t pad[2];
     float stops[16];       // speed-colour gradient, Color1..Color4 RGBA (fluidSimCPU.h:26-29)
 };

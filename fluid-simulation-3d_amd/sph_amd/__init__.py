@@ -29,6 +29,7 @@ SPH_ERR_BUDGET = -4
 SPH_ERR_STATE = -5
 SPH_ERR_NOT_FOUND = -6
 SPH_ERR_NONFINITE = -7
+SPH_ERR_DEVICE = -8
 
 # bucket keys of sph_create's table_size (sph_amd.h)
 TABLE_GRID = 0
@@ -77,7 +78,8 @@ REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_CANCELLED = 0, 1, 2, 3
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("max_running", ctypes.c_uint32),
                 ("initial_capacity", ctypes.c_uint32), ("dt", ctypes.c_float),
-                ("substeps", ctypes.c_int32), ("updates_per_tick", ctypes.c_int32)]
+                ("substeps", ctypes.c_int32), ("updates_per_tick", ctypes.c_int32),
+                ("max_capacity", ctypes.c_uint32)]
 
 
 class TickInfo(ctypes.Structure):
@@ -92,7 +94,8 @@ class ServerStats(ctypes.Structure):
                                                "particle_updates", "ticks", "capacity")] + \
                [(k, ctypes.c_double) for k in ("busy_s", "tick_ms_p50", "tick_ms_p99",
                                                "request_ms_p50", "request_ms_p99", "wait_ms_p50",
-                                               "wait_ms_p99")]
+                                               "wait_ms_p99")] + \
+               [("admission_failures", ctypes.c_uint64)]
 
 
 def _struct_dict(st):
@@ -150,6 +153,8 @@ def lib():
         "sph_profile_end": (i32, [H, p, p]),
         "sph_debug_spatial": (i32, [H, p, p, p, p, p, p]),
         "sph_debug_predicted": (i32, [H, p]),
+        "sph_debug_velocities": (i32, [H, p, p]),
+        "sph_debug_set_scan_spin": (i32, [H, u32]),
         "sph_debug_neighbours": (i32, [H, p, p, u64, p]),
         "sph_generate": (i32, [i32, u64, u32, p, f, f, f, p, p]),
         "sph_server_default_config": (None, [ctypes.POINTER(ServerConfig)]),
@@ -160,7 +165,7 @@ def lib():
         "sph_server_submit": (i32, [H, pp, u32, u32, p, p, u32, p]),
         "sph_server_tick": (i32, [H, ctypes.POINTER(TickInfo)]),
         "sph_server_status": (i32, [H, u64, p, p, p]),
-        "sph_server_fetch": (i32, [H, u64, p, p, p]),
+        "sph_server_fetch": (i32, [H, u64, u32, p, p, p]),
         "sph_server_cancel": (i32, [H, u64]),
         "sph_server_get_stats": (i32, [H, ctypes.POINTER(ServerStats)]),
         "sph_batch_create": (i32, [u32, p, p, p, ctypes.POINTER(H)]),
@@ -404,6 +409,11 @@ class Sim:
     def variants(self, d):
         _check(lib().sph_set_variants(self._h, _ptr(_variants_array(d))))
 
+    def debug_set_scan_spin(self, sleeps):
+        """Bound of the count scan's look-back wait (64-clock sleeps); 0 makes
+        every wait give up, forcing the SPH_ERR_DEVICE path."""
+        _check(lib().sph_debug_set_scan_spin(self._h, int(sleeps)))
+
     def count_nonfinite(self):
         c = np.zeros(1, np.uint64)
         _check(lib().sph_count_nonfinite(self._h, _ptr(c)))
@@ -431,6 +441,14 @@ class Sim:
         xp = np.zeros((self.n, 4), np.float32)
         _check(lib().sph_debug_predicted(self._h, _ptr(xp)))
         return xp
+
+    def debug_velocities(self):
+        """(v_ext, v_press): the last update's velocities after its external
+        force (cc:45) and after its pressure stage (cc:421), particle order."""
+        ve = np.zeros((self.n, 4), np.float32)
+        vp = np.zeros((self.n, 4), np.float32)
+        _check(lib().sph_debug_velocities(self._h, _ptr(ve), _ptr(vp)))
+        return ve, vp
 
     def debug_neighbours(self):
         offs = np.zeros(self.n + 1, np.uint64)
@@ -577,6 +595,8 @@ class Server:
     def submit(self, params, pos4, vel4, steps, table=0):
         pos = np.ascontiguousarray(pos4, np.float32)
         vel = np.ascontiguousarray(vel4, np.float32)
+        if pos.ndim != 2 or pos.shape[1] != 4 or pos.shape[0] == 0 or vel.shape != pos.shape:
+            raise ValueError(f"pos and vel must both have shape (n, 4), got {pos.shape} and {vel.shape}")
         rid = np.zeros(1, np.uint64)
         _check(lib().sph_server_submit(self._h, ctypes.byref(params), pos.shape[0], int(table),
                                        _ptr(pos), _ptr(vel), int(steps), _ptr(rid)))
@@ -595,10 +615,11 @@ class Server:
         return int(st[0]), int(done[0]), float(lat[0])
 
     def fetch(self, rid, n):
+        """Results of a finished request of n particles (the library checks n)."""
         pos = np.zeros((n, 4), np.float32)
         vel = np.zeros((n, 4), np.float32)
         dens = np.zeros((n, 2), np.float32)
-        _check(lib().sph_server_fetch(self._h, int(rid), _ptr(pos), _ptr(vel), _ptr(dens)))
+        _check(lib().sph_server_fetch(self._h, int(rid), int(n), _ptr(pos), _ptr(vel), _ptr(dens)))
         return pos, vel, dens
 
     def cancel(self, rid):

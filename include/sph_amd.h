@@ -41,6 +41,10 @@ extern "C" {
 #define SPH_ERR_STATE (-5)     /* call not valid in the current state */
 #define SPH_ERR_NOT_FOUND (-6) /* unknown request id */
 #define SPH_ERR_NONFINITE (-7) /* state holds NaN/Inf */
+#define SPH_ERR_DEVICE (-8)    /* a bounded device-side wait gave up during an
+                                  update: its results, and the state of every
+                                  scene it advanced, are invalid until new
+                                  state is uploaded or generated */
 
 /* Simulation settings.  Field <-> reference member (physicsWorld.h):
  *   smoothing_radius   interactionRadius       :97  (setInteractionRadius :58)
@@ -198,6 +202,18 @@ int sph_autotune(sph_handle h, float dt, int updates, double ms[3]);
  * the reference key, per-row 64-bit neighbour masks under the grid key. */
 int sph_get_variants(sph_handle h, int32_t v[13]);
 int sph_set_variants(sph_handle h, const int32_t v[13]);
+/* Test support, no reference counterpart: the bound of the count scan's
+ * look-back wait (the prefix sum behind the bucket table of UpdateSpatialLookup,
+ * cc:486-496), in sleeps of 64 clocks; 65536 by default, about 1.7 ms.  A
+ * workgroup waits there only for chunks of the scan taken before its own by
+ * workgroups already running, so a wait that reaches the bound means the
+ * device was held up elsewhere (time-sliced with other work): the update's
+ * bucket table is then incomplete, the next synchronising call (sph_sync,
+ * sph_step, sph_update, sph_download, the debug readbacks) returns
+ * SPH_ERR_DEVICE, and the state stays invalid until sph_upload (positions and
+ * velocities) or sph_initialize replaces it.  0 makes every wait give up at
+ * once, which forces that path from the host. */
+int sph_debug_set_scan_spin(sph_handle h, uint32_t sleeps);
 /* Number of non-finite position/velocity components in the current state. */
 int sph_count_nonfinite(sph_handle h, uint64_t *count);
 
@@ -213,6 +229,13 @@ int sph_debug_spatial(sph_handle h, uint32_t *hash_id, uint32_t *key_id,
 /* predictedPositions of the last update (physicsWorld.h:118), float4, w = 0,
  * particle order; SPH_ERR_STATE before the first update. */
 int sph_debug_predicted(sph_handle h, float *xp4);
+/* The velocity vector of the last update after its external-force stage
+ * (cc:45) and after its pressure stage (cc:62-69, 421), before viscosity:
+ * float4 (w unused) in particle order; either pointer may be NULL.  So the
+ * pressure term of each particle is v_press - v_ext and the viscosity term
+ * (cc:463) is the downloaded velocity - v_press, for particles that did not
+ * touch a wall (cc:90-105).  SPH_ERR_STATE before the first update. */
+int sph_debug_velocities(sph_handle h, float *v_ext4, float *v_press4);
 /* Neighbour sets of the last update as the reference's bucket walk sees them
  * (cc:331-363): for particle i, every j in the 27 cells whose hash matches and
  * whose squared distance is <= prefilter_r2, self included, in walk order.
@@ -298,6 +321,11 @@ typedef struct sph_server_config {
     float dt;                  /* frame dt of one step (default 1/120) */
     int32_t substeps;          /* Update calls per step (default 3) */
     int32_t updates_per_tick;  /* Update calls per tick (default 1: re-batch every substep) */
+    uint32_t max_capacity;     /* particle slots the device layout may grow to (0: 2^31).
+                                  Admissions that need more -- or that the device has no
+                                  memory for -- wait at the head of the queue while the
+                                  running requests go on (stats.admission_failures counts
+                                  them); with nothing running the tick fails SPH_ERR_OOM */
 } sph_server_config;
 
 typedef struct sph_server_tick_info {
@@ -311,6 +339,7 @@ typedef struct sph_server_stats {
     uint64_t used_bytes, particle_updates, ticks, capacity;
     double busy_s; /* wall time of ticks that advanced requests */
     double tick_ms_p50, tick_ms_p99, request_ms_p50, request_ms_p99, wait_ms_p50, wait_ms_p99;
+    uint64_t admission_failures; /* ticks whose admissions could not grow the layout (they waited) */
 } sph_server_stats;
 
 typedef struct sph_server_t *sph_server;
@@ -332,8 +361,10 @@ int sph_server_tick(sph_server s, sph_server_tick_info *info);
 int sph_server_status(sph_server s, uint64_t id, int32_t *state, uint64_t *updates_done,
                       double *latency_ms);
 /* Copies a finished request's positions, velocities and densities (any may be
- * NULL) and forgets the request. */
-int sph_server_fetch(sph_server s, uint64_t id, float *pos4, float *vel4, float *dens2);
+ * NULL; n float4, float4 and float2 entries) and forgets the request.  n must
+ * be the request's particle count (SPH_ERR_INVALID otherwise, nothing
+ * copied). */
+int sph_server_fetch(sph_server s, uint64_t id, uint32_t n, float *pos4, float *vel4, float *dens2);
 int sph_server_cancel(sph_server s, uint64_t id);
 int sph_server_get_stats(sph_server s, sph_server_stats *st);
 

@@ -7,7 +7,9 @@
 // getters return zero for an out-of-range index (physicsWorld.cc:149-182),
 // setters take effect at the next Update, the public `positions` and
 // `OutPositions` vectors hold the state after each Update (the reference's
-// render hand-off, fluidSimCPU.cc:104).  Vector types are layout-compatible
+// render hand-off, fluidSimCPU.cc:104), and positions written into
+// `positions` between Updates are the ones the next Update starts from, as
+// stage A reads them in the reference (physicsWorld.cc:47).  Vector types are layout-compatible
 // with glm::vec3 / glm::vec4 (three / four packed floats); define
 // SPH_AMD_VEC3 / SPH_AMD_VEC4 to use the application's own (e.g. glm) types.
 #pragma once
@@ -54,10 +56,13 @@ class FluidSimulation {
     }
 
     // physicsWorld.h:34 / cc:39-111: one Update on the GPU, then the host
-    // copies of positions, velocities and densities are refreshed.
+    // copies of positions, velocities and densities are refreshed.  Positions
+    // the caller wrote into `positions` since the last refresh go to the
+    // device first (only positions; velocities keep the device's values).
     void Update(float deltatime)
     {
         if (!h_) return;
+        push_positions();
         check(sph_update(h_, deltatime));
         pull();
     }
@@ -97,7 +102,16 @@ class FluidSimulation {
         return std::min(std::max(getSpeed(i), 0.0f), 1.5f) / 1.5f;
     }
 
-    // physicsWorld.h:45-50: stage times of the last Update in ms
+    // physicsWorld.h:45-50: stage times of the last Update in ms, measured
+    // with device events around each kernel.  Two of the reference's six
+    // stages have no kernel of their own:
+    //  - gravity (stage A, cc:41-50) reads the time of the separate predict
+    //    kernel, which runs only when the previous Update did not record this
+    //    one's buckets (the first Update, or after new state, settings or dt);
+    //    on every other Update stage A runs inside the previous Update's
+    //    viscosity kernel and this one's sort, so it reads 0;
+    //  - position + collision (stage F, cc:80-110) runs in the epilogue of the
+    //    viscosity kernel, so it reads 0 and getElapsedTimeViscosity includes it.
     double getElapsedTimeGravity() { return stage(0); }
     double getElapsedTimeSpatial() { return stage(1); }
     double getElapsedTimeDensity() { return stage(2); }
@@ -168,6 +182,22 @@ class FluidSimulation {
             positions[i] = vec3{pos4_[4 * i], pos4_[4 * i + 1], pos4_[4 * i + 2]};
             OutPositions[i] = vec4{pos4_[4 * i], pos4_[4 * i + 1], pos4_[4 * i + 2], pos4_[4 * i + 3]};
         }
+    }
+    // The caller's writes to `positions` (h:79) since the last pull, to the device.
+    void push_positions()
+    {
+        if (positions.size() != n_ || pos4_.size() != 4 * (size_t)n_) return;
+        bool changed = false;
+        for (uint32_t i = 0; i < n_ && !changed; ++i)
+            changed = positions[i].x != pos4_[4 * i] || positions[i].y != pos4_[4 * i + 1] ||
+                      positions[i].z != pos4_[4 * i + 2];
+        if (!changed) return;
+        for (uint32_t i = 0; i < n_; ++i) {
+            pos4_[4 * i] = positions[i].x;
+            pos4_[4 * i + 1] = positions[i].y;
+            pos4_[4 * i + 2] = positions[i].z;
+        }
+        check(sph_upload(h_, pos4_.data(), nullptr));
     }
     double stage(int q)
     {

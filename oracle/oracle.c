@@ -131,6 +131,12 @@ static int g_grid_key = 0;
 static uint32_t g_gdim[3];
 static int32_t g_glo[3];
 
+/* Reference sort (timing only): stage B2 as std::sort of 12-byte float
+ * records (oracle_sort.cpp) instead of the stable counting sort. */
+static int g_reference_sort = 0;
+void orc_std_sort_records(float *rec3, uint32_t n); /* oracle_sort.cpp */
+void orc_set_reference_sort(int on) { g_reference_sort = on != 0; }
+
 void orc_set_grid_key(const uint32_t *gdim, const int32_t *lo)
 {
     g_grid_key = gdim != NULL && lo != NULL;
@@ -195,13 +201,32 @@ int orc_stage_spatial(uint32_t n, uint32_t T, float h, const float *xp4, uint32_
     }
     for (uint32_t k = 0; k < T; ++k) start[k] = ORC_START_EMPTY;
 
-    /* B2, cc:484: sort by key -- restated as a stable counting sort */
-    for (uint32_t i = 0; i < n; ++i) cnt[kid[i] + 1]++;
-    for (uint32_t k = 0; k < T; ++k) cnt[k + 1] += cnt[k];
-    for (uint32_t i = 0; i < n; ++i) {
-        uint32_t pos = cnt[kid[i]]++;
-        sk[pos] = kid[i];
-        si[pos] = i;
+    if (g_reference_sort) {
+        /* B2, cc:480 + 484 as the reference runs it: {i, hash, key} as float
+         * records, std::sort by key, serially */
+        float *rec = (float *)malloc(sizeof(float) * 3 * (size_t)n);
+        if (!rec) { rc = -1; goto out; }
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+        for (int64_t i = 0; i < (int64_t)n; ++i) {
+            rec[3 * i + 0] = (float)(uint32_t)i;
+            rec[3 * i + 1] = (float)hid[i];
+            rec[3 * i + 2] = (float)kid[i];
+        }
+        orc_std_sort_records(rec, n);
+        for (uint32_t k = 0; k < n; ++k) {
+            si[k] = (uint32_t)rec[3 * k + 0];
+            sk[k] = (uint32_t)rec[3 * k + 2];
+        }
+        free(rec);
+    } else {
+        /* B2, cc:484: sort by key -- restated as a stable counting sort */
+        for (uint32_t i = 0; i < n; ++i) cnt[kid[i] + 1]++;
+        for (uint32_t k = 0; k < T; ++k) cnt[k + 1] += cnt[k];
+        for (uint32_t i = 0; i < n; ++i) {
+            uint32_t pos = cnt[kid[i]]++;
+            sk[pos] = kid[i];
+            si[pos] = i;
+        }
     }
 
     /* B3, cc:486-496 */
@@ -482,10 +507,13 @@ void orc64_pressure(uint32_t n, uint32_t T, const orc_params *p, double dt, cons
         double sp = 0.5 * (P + (rj - rho0) * k), snp = 0.5 * (Pn + rnj * kn);
         double a = d <= h ? -(h - d) * s2 : 0.0;
         double b = d <= h ? -(h - d) * (h - d) * s3 : 0.0;
+        /* scale: the terms with |P_i| + |P_j| in place of |P_i + P_j|, so a
+         * shared pressure that cancels keeps the size of its two inputs */
+        const double up = 0.5 * (fabs(P) + fabs((rj - rho0) * k)), unp = 0.5 * (fabs(Pn) + fabs(rnj * kn));
         for (int c = 0; c < 3; ++c) {
             double t1 = dir[c] * a * sp / rj, t2 = dir[c] * b * snp / rnj;
             F[c] += t1 + t2;
-            M[c] += fabs(t1) + fabs(t2);
+            M[c] += fabs(dir[c] * a) * up / fabs(rj) + fabs(dir[c] * b) * unp / fabs(rnj);
         }
         WALK_END
         for (int c = 0; c < 3; ++c) {

@@ -84,6 +84,15 @@ uint32_t orc_key(uint32_t hash, uint32_t table_size);
  * The sums of a step must not depend on which key is used.  Not thread-safe. */
 void orc_set_grid_key(const uint32_t *gdim, const int32_t *lo);
 
+/* Timing hook (CPU baseline), no effect on the other entry points' contracts
+ * beyond the within-bucket order: on != 0 runs stage B2 as the reference does
+ * -- std::sort (serial) of 12-byte float records {index, hash, key} by key
+ * (physicsWorld.cc:480, 484; compareByKey, physicsWorld.h:158-161) -- instead
+ * of the stable counting sort.  Keys and per-bucket index sets are the same;
+ * the within-bucket order, and so the summation order, is libstdc++'s.  Exact
+ * for n and keys below 2^24 (float records).  Not thread-safe. */
+void orc_set_reference_sort(int on);
+
 /* ---- stages of Update (physicsWorld.cc:39-111) over float4 (x,y,z,w) arrays ---- */
 /* Stage A, cc:42-48: vel += g*dt; xp = pos + vel*lookahead.  xp4[.w] = 0. */
 void orc_stage_predict(uint32_t n, const orc_params *p, float dt,
@@ -142,7 +151,10 @@ void orc64_density(uint32_t n, uint32_t table_size, const orc_params *p,
                    const uint32_t *sorted_idx, const uint32_t *hash_id,
                    const uint32_t *start, double *dens2);
 /* Pressure acceleration*dt in double: dv3[3i+c] = (F_c/rho_i)*dt; mag3 =
- * sum over pair terms of |term_c| / rho_i * dt (the scale of the sum). */
+ * the scale of the sum: over pair terms, |dir_c * dW| * (|P_i| + |P_j|) / 2
+ * / |rho_j| (and the near-pressure twin), / |rho_i| * dt.  A shared pressure
+ * (P_i + P_j) / 2 that cancels keeps the size of its inputs, whose FP32
+ * rounding it carries. */
 void orc64_pressure(uint32_t n, uint32_t table_size, const orc_params *p, double dt,
                     const float *xp4, const uint32_t *sorted_key,
                     const uint32_t *sorted_idx, const uint32_t *hash_id,

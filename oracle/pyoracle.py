@@ -58,6 +58,7 @@ def lib():
             "orc_hash_cell": (u32, [p]),
             "orc_key": (u32, [u32, u32]),
             "orc_set_grid_key": (None, [p, p]),
+            "orc_set_reference_sort": (None, [i32]),
             "orc_stage_predict": (None, [u32, pp, f, p, p, p]),
             "orc_stage_spatial": (i32, [u32, u32, f, p, p, p, p, p, p]),
             "orc_stage_density": (None, [u32, u32, pp, p, p, p, p, p, p]),
@@ -108,6 +109,12 @@ def from_sph(p):
 
 def set_threads(n):
     lib().orc_set_threads(int(n))
+
+
+def set_reference_sort(on):
+    """Timing hook: stage B2 as std::sort of 12-byte float records (the
+    reference's cc:484) instead of the stable counting sort."""
+    lib().orc_set_reference_sort(1 if on else 0)
 
 
 def W2(d, h): return lib().orc_W2(d, h)

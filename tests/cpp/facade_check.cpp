@@ -35,12 +35,20 @@ int main(int argc, char **argv)
     if (col.size() != 10000 || core.out_positions().size() != 10000) return 7;
     FILE *f = fopen(argv[1], "wb");
     if (!f) return 6;
-    for (uint32_t i = 0; i < 10000; ++i) {
-        const auto p = sim.getPosition(i), v = sim.getVelocity(i);
-        const float rec[12] = {p.x, p.y, p.z, v.x, v.y, v.z, sim.getDensity(i), sim.getNearDensity(i),
-                               col[i].x, col[i].y, col[i].z, col[i].w};
-        fwrite(rec, sizeof(float), 12, f);
-    }
+    auto dump = [&](const std::vector<sph_amd::FluidSimHIPCore::vec4> &c) {
+        for (uint32_t i = 0; i < 10000; ++i) {
+            const auto p = sim.getPosition(i), v = sim.getVelocity(i);
+            const float rec[12] = {p.x, p.y, p.z, v.x, v.y, v.z, sim.getDensity(i), sim.getNearDensity(i),
+                                   c[i].x, c[i].y, c[i].z, c[i].w};
+            fwrite(rec, sizeof(float), 12, f);
+        }
+    };
+    dump(col);
+    // the public positions vector is an input too (physicsWorld.h:79): lift
+    // the fluid by 0.05 and step once more from there
+    for (auto &p : sim.positions) p.y += 0.05f;
+    sim.Update(1.0f / 60.0f);
+    dump(core.colors());
     fclose(f);
     printf("facade ok: speed[0]=%g normalized=%g\n", sim.getSpeed(0), sim.getSpeedNormalzied(0));
     return 0;

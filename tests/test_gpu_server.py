@@ -4,8 +4,9 @@ around it; admission respects slots and the HBM budget."""
 import numpy as np
 import pytest
 
+import pyoracle as O
 import sph_amd as S
-from helpers import announce, assert_bitexact
+from helpers import announce, assert_bitexact, orc_params
 from sph_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -55,6 +56,16 @@ def test_requests_alone_equal_mixed():
         ref = _alone(*scenes[q], steps, table=777 if q == 2 else 0)
         for g, a, what in zip(results[q], ref, ("pos", "vel", "dens")):
             assert_bitexact(g, a, f"request {q} {what}")
+    # and the served results equal the oracle (requests admitted mid-run and
+    # queued behind the slots, one with its own bucket count)
+    dt = float(np.float32(np.float32(W.DT) / np.float32(W.SUBSTEPS)))
+    for q in (1, 2):
+        p, pos, vel = scenes[q]
+        pos_o, vel_o, dens_o = O.update(orc_params(p), dt, pos, vel, T=specs[q][0],
+                                        substeps=specs[q][1] * W.SUBSTEPS)
+        assert_bitexact(results[q][0][:, :3], pos_o[:, :3], f"request {q} pos vs oracle")
+        assert_bitexact(results[q][1][:, :3], vel_o[:, :3], f"request {q} vel vs oracle")
+        assert_bitexact(results[q][2], dens_o, f"request {q} dens vs oracle")
 
 
 def test_budget_admission_and_refusal():
@@ -109,3 +120,46 @@ def test_capacity_growth_keeps_results():
     for r, s_ in zip(res, sc):
         ref = _alone(*s_, 2)
         assert_bitexact(r[1], ref[1], "vel after growth")
+
+
+def test_failed_growth_leaves_admission_undone():
+    """Admissions that need more device slots than the layout may grow to
+    (max_capacity, as when device memory runs out) change nothing: the
+    request waits at the head of the queue with the budget uncharged, the
+    running request goes on; once it is done the queued one fits and runs,
+    and both equal their runs alone.  With nothing running, a request that
+    cannot fit fails the tick with SPH_ERR_OOM."""
+    a = _scene(40000, 51)
+    b = _scene(50000, 52)
+    c = _scene(70000, 53)
+    with S.Server(max_running=4, initial_capacity=1 << 16, max_capacity=1 << 16) as srv:
+        ia = srv.submit(a[0], a[1], a[2], 2)
+        srv.tick()
+        assert srv.status(ia)[0] == S.REQ_RUNNING
+        used = srv.stats()["used_bytes"]
+        ib = srv.submit(b[0], b[1], b[2], 2)
+        done_before = srv.status(ia)[1]
+        srv.tick()                                        # 90,000 particles > 65,536 slots
+        assert srv.status(ib)[0] == S.REQ_QUEUED
+        assert srv.status(ia)[1] == done_before + 1       # a went on
+        st = srv.stats()
+        assert st["admission_failures"] >= 1
+        assert st["used_bytes"] == used and st["admitted"] == 1 and st["queued"] == 1
+        _drain(srv, [ia, ib])                             # b runs once a has left
+        ra = srv.fetch(ia, 40000)
+        with pytest.raises(S.SphError) as e3:
+            srv.fetch(ia, 40000)                          # forgotten once fetched
+        assert e3.value.code == S.SPH_ERR_NOT_FOUND
+        with pytest.raises(S.SphError) as e4:
+            srv.fetch(ib, 49999)                          # wrong particle count: refused
+        assert e4.value.code == S.SPH_ERR_INVALID
+        rb = srv.fetch(ib, 50000)
+        ic = srv.submit(c[0], c[1], c[2], 1)              # larger than the layout can ever be
+        with pytest.raises(S.SphError) as e5:
+            srv.tick()
+        assert e5.value.code == S.SPH_ERR_OOM
+        assert srv.status(ic)[0] == S.REQ_QUEUED
+        srv.cancel(ic)
+        assert srv.stats()["queued"] == 0
+    assert_bitexact(ra[0], _alone(*a, 2)[0], "a beside a failed admission")
+    assert_bitexact(rb[1], _alone(*b, 2)[1], "b after waiting for room")

@@ -250,3 +250,30 @@ def test_update_composes_its_stages():
     np.testing.assert_array_equal(x, x4)
     np.testing.assert_array_equal(v, v4)
     np.testing.assert_array_equal(d, dens)
+
+
+def test_reference_sort_mode_keeps_keys_and_bucket_sets():
+    """The CPU baseline's stage B2 (std::sort of 12-byte float records, as
+    cc:480-484 run it) gives the same key sequence, start table and per-bucket
+    index sets as the stable order, and an Update close to the stable one
+    (only the within-bucket summation order differs)."""
+    p = O.params(h=0.2, rho0=1000.0, mu=0.05, gravity_on=1, gravity_scale=9.81, bounds=(2, 2, 2))
+    pos = _random_scene(3000, (1.2, 1.2, 1.2), 8)
+    vel = np.zeros_like(pos)
+    _, xp = O.predict(p, 0.01, pos, vel)
+    stable = O.spatial(xp, 500, p.h)
+    O.set_reference_sort(True)
+    try:
+        ref = O.spatial(xp, 500, p.h)
+        x_r, v_r, d_r = O.update(p, float(np.float32(1 / 360)), pos, vel, T=3000)
+    finally:
+        O.set_reference_sort(False)
+    np.testing.assert_array_equal(ref.sorted_key, stable.sorted_key)
+    np.testing.assert_array_equal(ref.start, stable.start)
+    for k in np.unique(stable.sorted_key):
+        sel = stable.sorted_key == k
+        assert set(ref.sorted_idx[sel].tolist()) == set(stable.sorted_idx[sel].tolist())
+    assert not np.array_equal(ref.sorted_idx, stable.sorted_idx)  # an unstable order indeed
+    x_s, v_s, d_s = O.update(p, float(np.float32(1 / 360)), pos, vel, T=3000)
+    np.testing.assert_allclose(d_r, d_s, rtol=1e-5)
+    np.testing.assert_allclose(x_r, x_s, rtol=1e-5, atol=1e-6)
