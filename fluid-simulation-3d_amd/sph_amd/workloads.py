@@ -75,6 +75,14 @@ def two_blocks(n=1 << 24, seed=2, h=0.2, steps=500, box=(40.0, 20.0, 40.0), spee
     return Scene("two_blocks_%d" % n, scene_params(h=h, box=box), n, n, pos, vel, steps)
 
 
+def two_blocks_plan(n=1 << 24, seed=2, h=0.2, steps=500, box=(40.0, 20.0, 40.0), speed=2.0):
+    """two_blocks without host arrays: the scene (pos, vel None) and the
+    generator arguments (kind, seed, box, spacing, jitter, speed) with which
+    sph_initialize writes the same bytes on the device."""
+    sc = Scene("two_blocks_%d" % n, scene_params(h=h, box=box), n, n, None, None, steps)
+    return {"scene": sc, "init": (INIT_TWO_BLOCKS, seed, box, 0.5 * h, 0.01 * h, speed)}
+
+
 @dataclass
 class BatchScenes:
     name: str
@@ -120,6 +128,14 @@ def serving_requests(duration_s, rate=20.0, seed=4, lo=8192, hi=2097152, steps_l
         steps = int(rng.integers(steps_lo, steps_hi + 1))
         kind = INIT_DAM_BREAK if rng.random() < 0.5 else INIT_CENTRED_BLOCK
         out.append((t, n, steps, kind, seed * 1000003 + len(out)))
+
+
+def serving_plan(n, kind, seed, h=0.2):
+    """serving_scene without host arrays: (params, (box, spacing, jitter,
+    speed)), the generator arguments of the request's initial state."""
+    L = (n * (0.5 * h) ** 3 / 0.4) ** (1.0 / 3.0)
+    box = (L, L, L)
+    return scene_params(h=h, box=box), (box, 0.5 * h, 0.01 * h, 0.0)
 
 
 def serving_scene(n, kind, seed, h=0.2):
