@@ -217,6 +217,33 @@ int sph_debug_set_scan_spin(sph_handle h, uint32_t sleeps);
 /* Number of non-finite position/velocity components in the current state. */
 int sph_count_nonfinite(sph_handle h, uint64_t *count);
 
+/* ---- one Update without allocation ----
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (serving loops and harnesses that forbid hipMalloc): one Update(dt)
+ * (cc:39-111) of one scene of n particles held in the caller's device memory,
+ * as sph_create(SPH_TABLE_GRID) + sph_upload + sph_update + sph_download would
+ * give it, bit for bit, with no device or host allocation: every intermediate
+ * lives in a fixed device array of the library (SPH_NOALLOC_SCRATCH_BYTES per
+ * GPU, held until the process ends).
+ *   pos4_in, vel4_in    device float4 arrays of n particles, particle order
+ *                       (read only)
+ *   pos4_out, vel4_out  device float4 arrays of n: the state after the
+ *                       Update (pos4.w = 0.34, OutPositions)
+ *   dens2_out           device float2 array of n (density, near density), or NULL
+ *   stream              hipStream_t to enqueue on (NULL: the null stream)
+ *   wait                1: return once done, reporting a device-side failure
+ *                       (SPH_ERR_DEVICE, as the handle path does); 0: return
+ *                       once enqueued (a failure is reported by the next
+ *                       waiting call)
+ * n must be in [1, SPH_NOALLOC_MAX_PARTICLES] (SPH_ERR_INVALID otherwise).
+ * Every call on one GPU shares that GPU's array: calls that overlap on one GPU
+ * (from several threads, or on streams not ordered with each other) race. */
+#define SPH_NOALLOC_MAX_PARTICLES 65536u
+#define SPH_NOALLOC_SCRATCH_BYTES 24647424u
+int sph_update_noalloc(const sph_params *p, uint32_t n, float dt, const float *pos4_in,
+                       const float *vel4_in, float *pos4_out, float *vel4_out, float *dens2_out,
+                       void *stream, int wait);
+
 /* ---- parity / debugging: the spatial lookup of the last update ----
  * UpdateSpatialLookup (cc:466-498): per particle hash (HashCell) and key
  * (GetKeyFromHash); per sorted slot key and particle index; per bucket the
