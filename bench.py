@@ -180,7 +180,7 @@ def roofline(workload, n, variants, grid, per, multi):
 # ---------------------------------------------------------------------------
 # GPU legs
 # ---------------------------------------------------------------------------
-def bench_batch(steps, warmup, plain_steps, latency_steps):
+def bench_batch(steps, warmup, plain_steps, latency_steps, plain=True):
     """Batched (one launch per kernel for all scenes, graph-replayed) vs plain
     (one handle per scene) on batch_256."""
     bs = W.batch()
@@ -215,6 +215,17 @@ def bench_batch(steps, warmup, plain_steps, latency_steps):
     b.close()
     log(f"[bench] batched {steps} steps in {el_b:.3f} s")
 
+    per = {STAGES[i]: float(stage_ms[i]) for i in range(5)}
+    v_b = total * steps / el_b
+    out = {"workload": bs.name, "scenes": len(bs.scenes), "particles": total,
+           "value": v_b, "ms_per_step": el_b * 1e3 / steps, "steps": steps,
+           "step_latency_ms": {"p50": float(np.percentile(lat, 50)),
+                               "p99": float(np.percentile(lat, 99)), "n": len(lat)},
+           "stage_ms_per_update": per, "gather_variants": variants,
+           "roofline": roofline(bs.name, total, variants, True, per, True),
+           "gather_kernels": gather_fractions(bs.name, total, variants, True, per, True)}
+    if not plain:
+        return out
     sims = []
     for sc in bs.scenes:
         sim = S.Sim(sc.params, sc.n, S.TABLE_GRID)
@@ -239,14 +250,9 @@ def bench_batch(steps, warmup, plain_steps, latency_steps):
     for sim in sims:
         sim.close()
     log(f"[bench] plain {plain_steps} steps in {el_p:.3f} s (one at a time {el_s:.3f} s)")
-    per = {STAGES[i]: float(stage_ms[i]) for i in range(5)}
-    v_b = total * steps / el_b
     v_p = total * plain_steps / el_p
     v_s = total * plain_steps / el_s
-    return {"workload": bs.name, "scenes": len(bs.scenes), "particles": total,
-            "value": v_b, "ms_per_step": el_b * 1e3 / steps, "steps": steps,
-            "step_latency_ms": {"p50": float(np.percentile(lat, 50)),
-                                "p99": float(np.percentile(lat, 99)), "n": len(lat)},
+    out.update({
             "plain_one_scene_per_launch": {
                 "value": v_p, "ms_per_step": el_p * 1e3 / plain_steps, "steps": plain_steps,
                 "note": "every scene on its own handle and stream, all enqueued before any is "
@@ -254,10 +260,8 @@ def bench_batch(steps, warmup, plain_steps, latency_steps):
             "plain_one_scene_at_a_time": {
                 "value": v_s, "ms_per_step": el_s * 1e3 / plain_steps, "steps": plain_steps,
                 "note": "the same launches, each scene's step finished before the next starts"},
-            "speedup_vs_plain": v_b / v_p, "speedup_vs_one_at_a_time": v_b / v_s,
-            "stage_ms_per_update": per, "gather_variants": variants,
-            "roofline": roofline(bs.name, total, variants, True, per, True),
-            "gather_kernels": gather_fractions(bs.name, total, variants, True, per, True)}
+            "speedup_vs_plain": v_b / v_p, "speedup_vs_one_at_a_time": v_b / v_s})
+    return out
 
 
 def bench_scene(sc, steps, warmup, key="grid", latency_steps=20, autotune=True, generated=None):
@@ -492,6 +496,8 @@ def main():
     ap.add_argument("--steps", type=int, default=100, help="timed batched steps (headline)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--plain-steps", type=int, default=5)
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the plain one-scene-per-launch paths (profiling runs)")
     ap.add_argument("--latency-steps", type=int, default=20)
     ap.add_argument("--scene-steps", type=int, default=100, help="timed steps of uniform_1m")
     ap.add_argument("--no-legs", action="store_true", help="headline only")
@@ -518,7 +524,8 @@ def main():
         ms = S.copy_calibration(args.copy_calibration_mib << 20, 3)
         log(f"[bench] copy calibration: {args.copy_calibration_mib} MiB in {ms:.4f} ms")
 
-    batch = bench_batch(args.steps, args.warmup, args.plain_steps, args.latency_steps)
+    batch = bench_batch(args.steps, args.warmup, args.plain_steps, args.latency_steps,
+                        plain=not args.no_plain)
     out = {
         "metric": METRIC,
         "value": batch["value"],
@@ -537,7 +544,7 @@ def main():
                    "h": "0.1-0.3", "viscosity": "0-0.2", "fill": 0.4, "bucket_key": "cell grid",
                    "dt": W.DT, "substeps": W.SUBSTEPS, "parallelism": "single GPU"},
         "step_latency_ms": batch["step_latency_ms"],
-        "batched_vs_single_scene": {
+        "batched_vs_single_scene": None if args.no_plain else {
             "batched": batch["value"],
             "plain_one_scene_per_launch": batch["plain_one_scene_per_launch"]["value"],
             "plain_one_scene_at_a_time": batch["plain_one_scene_at_a_time"]["value"],

@@ -134,6 +134,10 @@ def test_rollout_bitexact_dam_break_4k():
 
 
 def test_uniform_1m_tables_and_density():
+    """The 1,048,576-particle scene under the reference key with its 2^21
+    table, step 0: hashes, keys, sorted keys and order, the start table, every
+    particle's neighbour list (about 180M entries) and the densities equal the
+    oracle's."""
     sc = W.uniform_lower()
     dt = _sub_dt()
     announce("uniform_1m step-0 tables", sc.n, sc.table)
@@ -141,13 +145,21 @@ def test_uniform_1m_tables_and_density():
         sim.upload(sc.pos, sc.vel)
         sim.update(dt)
         sp_g = sim.debug_spatial()
+        offs_g, list_g = sim.debug_neighbours()
         _, _, dens_g = sim.download()
     op = orc_params(sc.params)
     _, xp = O.predict(op, dt, sc.pos, sc.vel)
     sp = O.spatial(xp, sc.table, op.h)
+    np.testing.assert_array_equal(sp_g["hash_id"], sp.hash_id)
+    np.testing.assert_array_equal(sp_g["key_id"], sp.key_id)
     np.testing.assert_array_equal(sp_g["sorted_key"], sp.sorted_key)
     np.testing.assert_array_equal(sp_g["sorted_idx"], sp.sorted_idx)
     np.testing.assert_array_equal(sp_g["start"], sp.start)
+    offs_o, list_o = O.neighbours(op, xp, sp)
+    announce("uniform_1m neighbour entries", int(offs_o[-1]))
+    np.testing.assert_array_equal(offs_g, offs_o)
+    assert np.array_equal(list_g, list_o), "neighbour lists differ"
+    del list_g, list_o
     dens_o = O.density(op, xp, sp)
     assert_bitexact(dens_g, dens_o, "density uniform_1m")
 
