@@ -136,7 +136,7 @@ def test_rollout_bitexact_dam_break_4k():
 def test_uniform_1m_tables_and_density():
     """The 1,048,576-particle scene under the reference key with its 2^21
     table, step 0: hashes, keys, sorted keys and order, the start table, every
-    particle's neighbour list (about 180M entries) and the densities equal the
+    particle's neighbour list (151,572,666 entries) and the densities equal the
     oracle's."""
     sc = W.uniform_lower()
     dt = _sub_dt()
