@@ -93,6 +93,10 @@ def kernel_name(stage, variants, grid, multi=False):
     return f"{base}<{m}, {b(v['work'])}, {b(v['list'])}, false, false>"
 
 
+# committed counter summaries are named for the workload's short name
+PROFILE_FILES = {"batch_256": "batch_256", "uniform_1048576": "uniform_1m"}
+
+
 def _profile(name):
     try:
         with open(os.path.join(PROFILES, name)) as f:
@@ -103,8 +107,8 @@ def _profile(name):
 
 def measured_traffic(workload, kernel):
     """HBM bytes per launch of `kernel` from the committed counter summary of
-    `workload` (profiles/<workload>_traffic.json), or None."""
-    t = _profile(f"{workload}_traffic.json")
+    `workload` (profiles/<short name>_traffic.json), or None."""
+    t = _profile(f"{PROFILE_FILES.get(workload, workload)}_traffic.json")
     if not t or t.get("workload") != workload:
         return None
     k = t.get("kernels", {}).get(kernel)
@@ -115,7 +119,7 @@ def valu_issue(workload, kernel, ms):
     """VALU issue fraction of `kernel` at duration `ms`, from its committed
     instruction count (rocprofv3 SQ_INSTS_VALU): instructions x 2 cycles (a
     wave64 VALU instruction on a 32-lane SIMD) / (1024 SIMDs x 2.4 GHz x ms)."""
-    t = _profile(f"{workload}_valu.json")
+    t = _profile(f"{PROFILE_FILES.get(workload, workload)}_valu.json")
     if not t or t.get("workload") != workload:
         return None
     k = t.get("kernels", {}).get(kernel)
@@ -490,6 +494,26 @@ def cpu_baseline_c0(threads, warmup_steps=5, steps=20):
                       f"float records"}
 
 
+LEGS = ["uniform_1m_grid", "uniform_1m_reference_key", "two_blocks_16m", "dam_break_16k"]
+
+
+def run_legs(names, args):
+    """The single-scene legs, each on its own handle."""
+    legs = {}
+    for name in names:
+        if name == "uniform_1m_grid":
+            legs[name] = bench_scene(W.uniform_lower(), args.scene_steps, args.warmup)
+        elif name == "uniform_1m_reference_key":
+            legs[name] = bench_scene(W.uniform_lower(), 20, args.warmup, key="reference")
+        elif name == "two_blocks_16m":
+            c2 = W.two_blocks_plan()
+            legs[name] = bench_scene(c2["scene"], 10, 2, generated=c2["init"])
+        elif name == "dam_break_16k":
+            c0 = W.dam_break(n=16384)
+            legs[name] = bench_scene(c0, c0.steps, args.warmup, autotune=False)
+    return legs
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -501,6 +525,10 @@ def main():
     ap.add_argument("--latency-steps", type=int, default=20)
     ap.add_argument("--scene-steps", type=int, default=100, help="timed steps of uniform_1m")
     ap.add_argument("--no-legs", action="store_true", help="headline only")
+    ap.add_argument("--legs", default=",".join(LEGS),
+                    help="comma-separated single-scene legs to run (default: all)")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched headline (profiling runs of the legs; value is then null)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-scenes", type=int, default=6)
     ap.add_argument("--no-serve", action="store_true")
@@ -524,6 +552,17 @@ def main():
         ms = S.copy_calibration(args.copy_calibration_mib << 20, 3)
         log(f"[bench] copy calibration: {args.copy_calibration_mib} MiB in {ms:.4f} ms")
 
+    legs_wanted = [x for x in args.legs.split(",") if x]
+    unknown = [x for x in legs_wanted if x not in LEGS]
+    if unknown:
+        print(f"unknown legs {unknown}; known: {LEGS}", file=sys.stderr)
+        return 2
+    if args.no_batch:
+        out = {"metric": METRIC, "value": None, "unit": "particle-steps/s", "n_gpus": 1,
+               "note": "batched headline skipped (--no-batch)",
+               "legs": {} if args.no_legs else run_legs(legs_wanted, args)}
+        print(json.dumps(out), flush=True)
+        return 0
     batch = bench_batch(args.steps, args.warmup, args.plain_steps, args.latency_steps,
                         plain=not args.no_plain)
     out = {
@@ -554,15 +593,7 @@ def main():
         "batch": batch,
     }
     if not args.no_legs:
-        legs = {}
-        legs["uniform_1m_grid"] = bench_scene(W.uniform_lower(), args.scene_steps, args.warmup)
-        legs["uniform_1m_reference_key"] = bench_scene(W.uniform_lower(), 20, args.warmup,
-                                                       key="reference")
-        c2 = W.two_blocks_plan()
-        legs["two_blocks_16m"] = bench_scene(c2["scene"], 10, 2, generated=c2["init"])
-        c0 = W.dam_break(n=16384)
-        legs["dam_break_16k"] = bench_scene(c0, c0.steps, args.warmup, autotune=False)
-        out["legs"] = legs
+        out["legs"] = run_legs(legs_wanted, args)
     if not args.no_serve:
         configured = bench_serve(args.serve_seconds, args.serve_rate, args.serve_slots, 200)
         plain = bench_serve_plain(args.serve_seconds, args.serve_rate, args.serve_slots, 200)

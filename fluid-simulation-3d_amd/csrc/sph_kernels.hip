@@ -714,7 +714,9 @@ __device__ __forceinline__ void walk_hash(const SceneDesc &S, const float4 *__re
 // stride made lanes eight records apart collide.
 constexpr int KBG = 4;                          // records per batch of the grid walk
 constexpr uint32_t kStageCap = 120;             // staged records per wave and row
-constexpr uint32_t kStageBuf = kStageCap + KBG; // + padding: a batch may read KBG-1 past the end
+// floats per plane: the cap + padding (a batch may read KBG-1 past the end)
+constexpr uint32_t kStageBuf = 144;
+static_assert(kStageBuf >= kStageCap + KBG, "staging planes");
 
 // Slots [b, e) of a grid row: the buckets of the row's cells dz = -1, 0, 1
 // (float hashes g0, g1, g2), in that order.  A record passes when d2 < thr
@@ -815,19 +817,46 @@ __device__ __forceinline__ uint64_t walk_bits(const float *sp, uint32_t len, con
     return len >= 64u ? mask : mask & ((1ull << len) - 1ull);
 }
 
-// Neighbour masks (grid key): for sorted slot k and row r, the run start
-// rb (base) and a 64-bit mask of the run's slots that passed (self
-// included); kept in the neighbour-row buffer as masks[9][n] (uint2) then
-// bases[9][n]; ok[k] = 0 when a row could not be recorded (a run longer than
-// 64 slots, or a seam row), and the consumer then walks that particle; else
-// ok[k] = kMasksOk | the bit set of the rows whose mask is not empty.
-constexpr uint32_t kMasksOk = 1u << 9;
+// Neighbour masks (grid key, row mode).  The density pass records, for
+// sorted slot k, which slots of its nine row runs passed the walk's tests
+// (self included), as one bitstream: row r's run [rb_r, rb_r + L_r) gives L_r
+// bits, bit o set when slot rb_r + o passed, the rows in walk order, so row r
+// starts at bit off_r = L_0 + ... + L_{r-1}.  The neighbour-row buffer holds
+// bits[3][n] (uint4: the 384-bit stream; the third word only written, and
+// read, when the stream is longer than 256 bits) and meta[n] (uint2: the nine
+// lengths, 7 bits each, as a 64-bit word whose top bit, kMaskUnusable, marks
+// an unusable record).  A record is unusable -- the consumers walk that
+// particle instead -- when a row's three buckets are not consecutive (the z
+// wrap seam), a run exceeds 64 slots or the runs exceed 384 bits in all.
+// Otherwise a consumer re-derives each rb_r as the walk found it, from the
+// particle's cell (its position is unchanged between the passes) and the
+// bucket bounds (a table the wave's lanes share in cache), so the record is
+// 40 bytes (56 past 256 bits).  A lattice of spacing h / 2 gives 216 bits.
+constexpr uint32_t kMaskBitsMax = 384;
+constexpr uint32_t kMaskWords = kMaskBitsMax / 32;
+constexpr uint64_t kMaskUnusable = 1ull << 63;
 
 struct MaskOut {
-    uint32_t *buf;  // neighbour-row buffer (>= 27 n words)
-    uint32_t *ok;   // [n]
+    uint32_t *buf;    // neighbour-row buffer (>= 14 n words)
     uint32_t n;
+    uint32_t *words;  // this lane's stream words in LDS (stride 64)
+    uint64_t lens;    // out: the nine run lengths, 7 bits each
 };
+
+// stream |= mask << off on this lane's LDS words (stride 64).  The mask has
+// no bit at or past its run's length L and off + L <= 384, so every word
+// touched lies in the stream.
+__device__ __forceinline__ void put_bits(uint32_t *mw, uint32_t off, uint64_t mask)
+{
+    const uint32_t q = off >> 5, s = off & 31u;
+    const uint32_t lo = (uint32_t)mask, hi = (uint32_t)(mask >> 32);
+    const uint32_t p0 = lo << s;
+    const uint32_t p1 = s ? (hi << s) | (lo >> (32u - s)) : hi;
+    const uint32_t p2 = s ? hi >> (32u - s) : 0u;
+    if (p0) mw[q * 64] |= p0;
+    if (p1) mw[(q + 1) * 64] |= p1;
+    if (p2) mw[(q + 2) * 64] |= p2;
+}
 
 // DEFER (density): where a row's run fits the 64-bit mask, the walk over the
 // run only records the mask, and the body then runs over the mask's slots
@@ -836,7 +865,8 @@ struct MaskOut {
 // neighbour instead of one per tested candidate batch.
 // MASK_ONLY (with MASKS): the walk records the masks and calls no body; seam
 // rows are not walked (they make the particle's masks unusable anyway).
-// Returns the value written to ok[k] (0 without MASKS).
+// MASKS: the record is written to mo->buf, and mo->lens is set;
+// returns 1 when the record is usable (0 without MASKS).
 template <bool SKIP_SELF, bool AUX, bool MASKS = false, bool DEFER = false, bool MASK_ONLY = false,
           typename Body>
 __device__ __forceinline__ uint32_t walk_grid(const SceneDesc &S, const float4 *__restrict__ xps,
@@ -844,21 +874,28 @@ __device__ __forceinline__ uint32_t walk_grid(const SceneDesc &S, const float4 *
                                           const uint32_t *__restrict__ start,
                                           const uint32_t *__restrict__ endt, uint32_t k,
                                           const float4 &pi, float4 *stage, bool clean, Body &&body,
-                                          const MaskOut &mo = MaskOut{nullptr, nullptr, 0})
+                                          MaskOut *mo = nullptr)
 {
     const int cx = cell_coord(pi.x, S.h), cy = cell_coord(pi.y, S.h), cz = cell_coord(pi.z, S.h);
     const float thr = S.thr;
     const uint32_t gy = S.gdim[1], gz = S.gdim[2], key_off = S.key_off;
     bool masks_ok = true;
-    uint32_t rowbits = 0;  // rows whose mask is not empty
+    uint32_t moff = 0;   // MASKS: bits recorded so far
+    uint64_t lens = 0;   // MASKS: the run lengths, 7 bits per row
+    uint32_t *mw = MASKS ? mo->words : nullptr;
+    if (MASKS)
+#pragma unroll
+        for (uint32_t w = 0; w < kMaskWords; ++w) mw[w * 64] = 0u;
     const uint32_t ux = (uint32_t)cx - (uint32_t)S.cell_lo[0];
     const uint32_t uy = (uint32_t)cy - (uint32_t)S.cell_lo[1];
     const uint32_t uz = (uint32_t)cz - (uint32_t)S.cell_lo[2];
+    // the particle's 27 cells lie inside the grid without wrapping
+    const bool interior = ux - 1u < S.gdim[0] - 2u && uy - 1u < gy - 2u && uz - 1u < gz - 2u;
     // No hash test needed: when no particle of the scene has its cell outside
     // the grid's box (clean) and this particle's 27 cells are all inside it,
     // the key is injective on the occupied cells, so each of those cells owns
     // its bucket and every record of a row's run belongs to the row.
-    const bool inner = clean && ux - 1u < S.gdim[0] - 2u && uy - 1u < gy - 2u && uz - 1u < gz - 2u;
+    const bool inner = clean && interior;
     const bool nohash = __ballot(inner) == __ballot(1);
     uint32_t wz[3];
 #pragma unroll
@@ -976,11 +1013,14 @@ __device__ __forceinline__ uint32_t walk_grid(const SceneDesc &S, const float4 *
                 }
             }
             if (MASKS) {
-                masks_ok = masks_ok && re - rb <= 64u;
-                rowbits |= (mask != 0 ? 1u : 0u) << r;
-                reinterpret_cast<uint2 *>(mo.buf)[(size_t)r * mo.n + k] =
-                    make_uint2((uint32_t)mask, (uint32_t)(mask >> 32));
-                mo.buf[(size_t)(18 + r) * mo.n + k] = rb;
+                const uint32_t L = re - rb;
+                if (L <= 64u && moff + L <= kMaskBitsMax) {
+                    put_bits(mw, moff, mask);
+                    lens |= (uint64_t)L << (7 * r);
+                    moff += L;
+                } else {
+                    masks_ok = false;
+                }
             }
         } else if (!MASK_ONLY) {
             masks_ok = false;
@@ -998,9 +1038,18 @@ __device__ __forceinline__ uint32_t walk_grid(const SceneDesc &S, const float4 *
             masks_ok = false;
         }
     }
-    const uint32_t ok = masks_ok ? kMasksOk | rowbits : 0u;
-    if (MASKS) mo.ok[k] = ok;
-    return MASKS ? ok : 0u;
+    if (!MASKS) return 0u;
+    const bool usable = masks_ok;
+    uint4 *bits = reinterpret_cast<uint4 *>(mo->buf);
+    mo->lens = lens;
+    const uint64_t meta = lens | (usable ? 0ull : kMaskUnusable);
+    reinterpret_cast<uint2 *>(mo->buf)[6 * (size_t)mo->n + k] = make_uint2((uint32_t)meta, (uint32_t)(meta >> 32));
+    if (usable) {
+        bits[k] = make_uint4(mw[0], mw[64], mw[128], mw[192]);
+        bits[(size_t)mo->n + k] = make_uint4(mw[256], mw[320], mw[384], mw[448]);
+        if (moff > 256u) bits[2 * (size_t)mo->n + k] = make_uint4(mw[512], mw[576], mw[640], mw[704]);
+    }
+    return usable ? 1u : 0u;
 }
 
 // Consumer side of the neighbour masks: body(m, ox, oy, oz, d2, aux[m]) for
@@ -1080,87 +1129,106 @@ __device__ __forceinline__ void take_masks(const float4 *__restrict__ xps, const
     }
 }
 
-// The mask consumers (pressure, viscosity).  The rows' masks and bases are
-// loaded at once into the wave's LDS table `tab` (rows x 64 lanes, 3 words
-// each), empty rows left out, so every table row holds at least one slot.
-// With SKIP_SELF the particle's own slot k is cleared from row 4 (dx = dy =
-// 0), the only row whose mask can hold it: it is a slot of the particle's own
-// cell.
-constexpr uint32_t kMaskTabWords = 9 * 64 * 3;  // per wave
-
+// Decodes a usable record and runs the body over its slots with take_masks.
+// The stream words are in this lane's LDS column (mw, stride 64).  Each
+// row's first slot rb_r is the start of the row's first bucket, found from
+// the particle's cell as walk_grid finds it (a usable record has no z seam,
+// so the row's buckets are consecutive).  All nine rows are cut from the
+// stream first (row r: L_r bits from bit off_r), then the non-empty ones go
+// to this lane's LDS column of the mask table (tab: entry t's mask at words
+// 2t, 2t + 1 and its first slot at word 18 + t, stride 64), which may lie
+// over the stream words: they are all read by then.  The take loop's
+// next_row is then one table read.  With SKIP_SELF the particle's own slot
+// k is cleared from row 4 (dx = dy = 0), its own cell's run.
+constexpr uint32_t kMaskTabWords = 27 * 64;  // per wave
 template <int NB, bool SKIP_SELF, int AUXN, typename Body>
-__device__ __forceinline__ void gather_masks(const float4 *__restrict__ xps,
-                                             const float4 *__restrict__ aux,
-                                             const uint32_t *__restrict__ buf, uint32_t n,
-                                             uint32_t k, const float4 &pi, uint32_t *tab,
-                                             Body &&body)
+__device__ __forceinline__ void gather_bits(const SceneDesc &S, const float4 *__restrict__ xps,
+                                            const float4 *__restrict__ aux,
+                                            const uint32_t *__restrict__ bounds, uint32_t k,
+                                            const float4 &pi, uint64_t lens, const uint32_t *mw,
+                                            uint32_t *tab, Body &&body)
 {
-    const uint2 *mk = reinterpret_cast<const uint2 *>(buf);
-    const uint32_t *bs = buf + 18 * (size_t)n;
-    const uint32_t lane = threadIdx.x & 63u;
-    uint2 *tm = reinterpret_cast<uint2 *>(tab);  // [9][64] masks
-    uint32_t *tb = tab + 9 * 64 * 2;             // [9][64] bases
-    uint32_t rows = 0;                           // non-empty rows in the table
+    const uint32_t gy = S.gdim[1], gz = S.gdim[2];
+    const uint32_t ux = (uint32_t)cell_coord(pi.x, S.h) - (uint32_t)S.cell_lo[0];
+    const uint32_t uy = (uint32_t)cell_coord(pi.y, S.h) - (uint32_t)S.cell_lo[1];
+    const uint32_t uz = (uint32_t)cell_coord(pi.z, S.h) - (uint32_t)S.cell_lo[2];
+    const uint32_t wz0 = grid_wrap(uz - 1u, gz, S.gmod[2]);
+    uint32_t wx[3], wy[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        wx[d] = grid_wrap(ux + (uint32_t)(d - 1), S.gdim[0], S.gmod[0]);
+        wy[d] = grid_wrap(uy + (uint32_t)(d - 1), gy, S.gmod[1]);
+    }
+    uint32_t rb[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) rb[r] = bounds[S.key_off + (wx[r / 3] * gy + wy[r % 3]) * gz + wz0];
+    uint64_t rm[9];
+    uint32_t off = 0;
 #pragma unroll
     for (int r = 0; r < 9; ++r) {
-        uint2 w = mk[(size_t)r * n + k];
-        const uint32_t b = bs[(size_t)r * n + k];
-        if (SKIP_SELF && r == 4) {
-            const uint32_t o = k - b;
-            w.x &= o < 32u ? ~(1u << o) : ~0u;
-            w.y &= o - 32u < 32u ? ~(1u << (o - 32u)) : ~0u;
-        }
-        tm[rows * 64 + lane] = w;
-        tb[rows * 64 + lane] = b;
-        rows += (w.x | w.y) != 0u ? 1u : 0u;
+        const uint32_t L = (uint32_t)(lens >> (7 * r)) & 127u;
+        // words past the stream's end only feed bits that the length mask
+        // drops (off + L <= 384, and off may be 384 when L = 0): clamp
+        const uint32_t q = off >> 5, sh = off & 31u;
+        const uint32_t w0 = mw[min(q, kMaskWords - 1u) * 64];
+        const uint32_t w1 = mw[min(q + 1u, kMaskWords - 1u) * 64];
+        const uint32_t w2 = mw[min(q + 2u, kMaskWords - 1u) * 64];
+        const uint32_t lo = sh ? (w0 >> sh) | (w1 << (32u - sh)) : w0;
+        const uint32_t hi = sh ? (w1 >> sh) | (w2 << (32u - sh)) : w1;
+        rm[r] = (((uint64_t)hi << 32) | lo) & (L >= 64u ? ~0ull : (1ull << L) - 1ull);
+        off += L;
     }
-    uint32_t r = 0;  // next table row
+    if (SKIP_SELF) rm[4] &= ~(1ull << (k - rb[4]));
+    uint32_t rows = 0;
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        tab[(2 * rows) * 64] = (uint32_t)rm[r];
+        tab[(2 * rows + 1) * 64] = (uint32_t)(rm[r] >> 32);
+        tab[(18 + rows) * 64] = rb[r];
+        rows += rm[r] != 0 ? 1u : 0u;
+    }
+    uint32_t t = 0;  // next table entry
     take_masks<NB, AUXN>(xps, aux, k, pi,
-                        [&](uint64_t &mask, uint32_t &base) {
-                            if (r < rows) {
-                                const uint2 w = tm[r * 64 + lane];
-                                mask = ((uint64_t)w.y << 32) | w.x;
-                                base = tb[r * 64 + lane];
-                                ++r;
-                            }
-                        },
-                        body);
+                         [&](uint64_t &mask, uint32_t &base) {
+                             if (t < rows) {
+                                 mask = (uint64_t)tab[(2 * t) * 64] | ((uint64_t)tab[(2 * t + 1) * 64] << 32);
+                                 base = tab[(18 + t) * 64];
+                                 ++t;
+                             }
+                         },
+                         body);
 }
 
-// Density over the masks it has just recorded (MASK_ONLY walk): the rows are
-// read back from the neighbour-row buffer, written by this work-item during
-// the walk, one row ahead of the one being taken, the non-empty ones picked
-// by `rowbits` (bit r: row r holds a slot).  The wave's body passes then
-// follow its largest total neighbour count rather than the sum over rows of
-// each row's largest count.
-template <int NB, typename Body>
-__device__ __forceinline__ void gather_own_masks(const float4 *__restrict__ xps,
-                                                 const uint32_t *__restrict__ buf, uint32_t n,
-                                                 uint32_t k, const float4 &pi, uint32_t rowbits,
-                                                 Body &&body)
+// The mask consumers (pressure, viscosity): the record's stream words go to
+// this lane's LDS column of the wave's table (tab), then gather_bits.
+template <int NB, bool SKIP_SELF, int AUXN, typename Body>
+__device__ __forceinline__ void gather_record(const SceneDesc &S, const float4 *__restrict__ xps,
+                                              const float4 *__restrict__ aux,
+                                              const uint32_t *__restrict__ bounds,
+                                              const uint32_t *__restrict__ buf, uint32_t n, uint32_t k,
+                                              const float4 &pi, uint64_t lens, uint32_t *tab, Body &&body)
 {
-    const uint2 *mk = reinterpret_cast<const uint2 *>(buf);
-    const uint32_t *bs = buf + 18 * (size_t)n;
-    // one row loaded ahead of the one handed out
-    uint2 pw = make_uint2(0u, 0u);
-    uint32_t pb = 0;
-    auto load_ahead = [&]() {
-        pw = make_uint2(0u, 0u);
-        if (rowbits != 0) {
-            const uint32_t r = (uint32_t)__builtin_ctz(rowbits);
-            rowbits &= rowbits - 1;
-            pw = mk[(size_t)r * n + k];
-            pb = bs[(size_t)r * n + k];
-        }
-    };
-    load_ahead();
-    take_masks<NB, 0>(xps, nullptr, k, pi,
-                      [&](uint64_t &mask, uint32_t &base) {
-                          mask = ((uint64_t)pw.y << 32) | pw.x;
-                          base = pb;
-                          load_ahead();
-                      },
-                      body);
+    const uint4 *bits = reinterpret_cast<const uint4 *>(buf);
+    const uint4 a = bits[k], c = bits[(size_t)n + k];
+    tab[0] = a.x;
+    tab[64] = a.y;
+    tab[128] = a.z;
+    tab[192] = a.w;
+    tab[256] = c.x;
+    tab[320] = c.y;
+    tab[384] = c.z;
+    tab[448] = c.w;
+    uint32_t total = 0;
+#pragma unroll
+    for (int r = 0; r < 9; ++r) total += (uint32_t)(lens >> (7 * r)) & 127u;
+    if (total > 256u) {
+        const uint4 e = bits[2 * (size_t)n + k];
+        tab[512] = e.x;
+        tab[576] = e.y;
+        tab[640] = e.z;
+        tab[704] = e.w;
+    }
+    gather_bits<NB, SKIP_SELF, AUXN>(S, xps, aux, bounds, k, pi, lens, tab, tab, body);
 }
 
 template <bool GRID, bool SKIP_SELF, bool AUX, bool MASKS = false, bool DEFER = false,
@@ -1170,7 +1238,7 @@ __device__ __forceinline__ uint32_t walk(const SceneDesc &S, const float4 *__res
                                      const uint32_t *__restrict__ start,
                                      const uint32_t *__restrict__ endt, uint32_t k,
                                      const float4 &pi, float4 *stage, bool clean, Body &&body,
-                                     const MaskOut &mo = MaskOut{nullptr, nullptr, 0})
+                                     MaskOut *mo = nullptr)
 {
     if constexpr (GRID)
         return walk_grid<SKIP_SELF, AUX, MASKS, DEFER, MASK_ONLY>(S, xps, aux, start, endt, k, pi, stage, clean,
@@ -1197,8 +1265,8 @@ __device__ __forceinline__ uint32_t walk(const SceneDesc &S, const float4 *__res
 //   listed ones.  Without LIST the body runs inside the walk.
 // NBR (row mode): the density pass records each particle's neighbours and
 //   pressure and viscosity take them from that record instead of walking:
-//   64-entry neighbour rows under the reference key (gather_row), per-row
-//   64-bit neighbour masks under the grid key (gather_masks).
+//   64-entry neighbour rows under the reference key (gather_row), a 48-byte
+//   record of each particle's row masks under the grid key (gather_record).
 // GRID: the cell-grid bucket key (walk_grid), sorted slots in grid order.
 // Which work-item handles a particle, and whether the body runs in the walk,
 // from the list or from the row-mode record, never changes a result.
@@ -1227,7 +1295,7 @@ __device__ __forceinline__ void gather(const SceneDesc &S, const float4 *__restr
                                        const uint32_t *__restrict__ start,
                                        const uint32_t *__restrict__ endt, uint32_t *lst,
                                        float4 *stage, bool clean, uint32_t k, const float4 &pi,
-                                       Body &&body, const MaskOut &mo = MaskOut{nullptr, nullptr, 0})
+                                       Body &&body, MaskOut *mo = nullptr)
 {
     if (!LIST) {
         walk<GRID, SKIP_SELF, AUX, MASKS>(S, xps, aux, start, endt, k, pi, stage, clean, body, mo);
@@ -1276,6 +1344,18 @@ __device__ __forceinline__ void gather(const SceneDesc &S, const float4 *__restr
     }
 }
 
+// LDS words per wave of a gather kernel past its lists: the staging planes,
+// with neighbour masks (density) the stream words and the mask table over
+// them, or (mask consumers) the table alone.
+__host__ __device__ constexpr uint32_t gather_wave_words(bool grid, bool nbr, bool consumer)
+{
+    return consumer ? kMaskTabWords
+                    : !grid ? 0u
+                            : !nbr ? 4 * kStageBuf
+                                   : (4 * kStageBuf + kMaskWords * 64 > kMaskTabWords ? 4 * kStageBuf + kMaskWords * 64
+                                                                                       : kMaskTabWords);
+}
+
 #define SPH_GATHER_PROLOGUE()                                                       \
     extern __shared__ uint32_t lds_list[];                                          \
     const uint32_t i = gather_index<WORK, GRID>();                                  \
@@ -1285,18 +1365,20 @@ __device__ __forceinline__ void gather(const SceneDesc &S, const float4 *__restr
     const SceneDesc &S = scene_of_slot<MULTI>(sd, scene_of, k);                     \
     const float4 pi = xps[k];                                                       \
     uint32_t *lst = lds_list + threadIdx.x;                                         \
-    /* GRID: each wave's row-staging buffer follows the lists */                    \
-    /* GRID: each wave's row-staging buffer follows the lists; the density   */  \
-    /* pass (CONSUMER false) stages rows, the mask consumers do not (their  */  \
-    /* rare fallback walk reads global memory) and hold a mask table instead */ \
-    float4 *stage = (GRID && !CONSUMER)                                            \
-                        ? reinterpret_cast<float4 *>(lds_list + (LIST ? kListMax * blockDim.x : 0)) + \
-                              (threadIdx.x >> 6) * kStageBuf                        \
-                        : nullptr;                                                  \
-    uint32_t *mtab = lds_list + (threadIdx.x >> 6) * kMaskTabWords;                \
+    /* GRID: each wave's LDS follows the lists (gather_wave_words words).    */  \
+    /* The walking passes (CONSUMER false) stage rows there; with neighbour */  \
+    /* masks the density pass's stream words (kMaskWords x 64) follow the   */  \
+    /* staging, and its mask table lies over both.  The mask consumers hold */  \
+    /* only the table (their rare fallback walk reads global memory).       */  \
+    uint32_t *wave_lds = lds_list + (LIST ? kListMax * blockDim.x : 0) +            \
+                         (threadIdx.x >> 6) * gather_wave_words(GRID, NBR, CONSUMER); \
+    float4 *stage = (GRID && !CONSUMER) ? reinterpret_cast<float4 *>(wave_lds) : nullptr; \
+    uint32_t *mwords = wave_lds + 4 * kStageBuf + (threadIdx.x & 63u);            \
+    uint32_t *mtab = wave_lds + (threadIdx.x & 63u);                               \
     /* grid key: no particle of this scene has its cell outside the grid's box */ \
     const bool clean = GRID && sflags[S.index] == 0u;                              \
     (void)mtab;                                                                     \
+    (void)mwords;                                                                   \
     (void)clean;                                                                    \
     (void)g;                                                                        \
     (void)lst
@@ -1378,21 +1460,30 @@ __global__ __launch_bounds__(256, MULTI ? 6 : 8) void k_density(SPH_GATHER_PARAM
             ++c;
         }
     };
-    // MASKS: the walk records the neighbour masks and the bodies then run over
-    // all of them; a particle whose masks are unusable (a run over 64 slots,
-    // a seam row) walks again with the body.  Grid key without masks or the
-    // LDS list: the body runs over each row's mask as the walk goes (DEFER).
+    // MASKS: the walk records the neighbour masks (the stream in this lane's
+    // LDS column, the record in the neighbour-row buffer) and the bodies then
+    // run over all of them, decoded from LDS into the mask table, which lies
+    // over the wave's staging planes (free once every lane's walk is done);
+    // a particle whose record is unusable walks again with the body, from
+    // global memory (other lanes of the wave hold their tables there).
+    // Grid key without masks or the LDS list: the body runs over each row's
+    // mask as the walk goes (DEFER).
+    MaskOut mo{nbr, ed->n_total, mwords, 0};
     if (GRID && !LIST && MASKS) {
         const uint32_t ok = walk<GRID, false, false, true, false, true>(
             S, xps, nullptr, start, endt, k, pi, stage, clean,
-            [](uint32_t, float, float, float, float, const float4 &) {}, MaskOut{nbr, ncnt, ed->n_total});
-        if (ok) gather_own_masks<4>(xps, nbr, ed->n_total, k, pi, ok & (kMasksOk - 1u), body);
-        else walk<GRID, false, false>(S, xps, nullptr, start, endt, k, pi, stage, clean, body);
+            [](uint32_t, float, float, float, float, const float4 &) {}, &mo);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (ok)
+            gather_bits<4, false, 0>(S, xps, nullptr, start, k, pi, mo.lens, mwords, mtab, body);
+        else walk<GRID, false, false>(S, xps, nullptr, start, endt, k, pi, nullptr, clean, body);
     } else if (GRID && !LIST)
         walk<GRID, false, false, false, true>(S, xps, nullptr, start, endt, k, pi, stage, clean, body);
     else
         gather<GRID, LIST, false, false, MASKS>(S, xps, nullptr, start, endt, lst, stage, clean, k, pi, body,
-                                                MaskOut{nbr, ncnt, ed->n_total});
+                                                MASKS ? &mo : nullptr);
     if (ROWS) ncnt[i] = c;
     // pressures of cc:371-372 / 407-408, computed once per particle
     // the pressure pass divides by these densities: their reciprocals ride along
@@ -1509,7 +1600,10 @@ __global__ __launch_bounds__(256) void k_pressure(SPH_GATHER_PARAMS, const float
         }
     };
     if (NBR && GRID) {
-        if (ncnt[k]) gather_masks<KB, true, 4>(xps, dens, nbr, ed->n_total, k, pi, mtab, body);
+        const uint2 mt2 = reinterpret_cast<const uint2 *>(nbr)[6 * (size_t)ed->n_total + k];
+        const uint64_t meta = (uint64_t)mt2.x | ((uint64_t)mt2.y << 32);
+        if (!(meta & kMaskUnusable))
+            gather_record<KB, true, 4>(S, xps, dens, start, nbr, ed->n_total, k, pi, meta, mtab, body);
         else gather<GRID, false, true, true>(S, xps, dens, start, endt, lst, stage, clean, k, pi, body);
     } else if (NBR) {
         gather_row(S, xps, dens, start, endt, nbr + (size_t)i * kNbrMax, ncnt[i], k, pi, body);
@@ -1557,7 +1651,10 @@ __global__ __launch_bounds__(256, MULTI ? 4 : 5) void k_viscosity_integrate(SPH_
         fz += (vj.z - vi.z) * w;
     };
     if (NBR && GRID) {
-        if (ncnt[k]) gather_masks<4, true, 3>(xps, vs2, nbr, ed->n_total, k, pi, mtab, body);
+        const uint2 mt2 = reinterpret_cast<const uint2 *>(nbr)[6 * (size_t)ed->n_total + k];
+        const uint64_t meta = (uint64_t)mt2.x | ((uint64_t)mt2.y << 32);
+        if (!(meta & kMaskUnusable))
+            gather_record<4, true, 3>(S, xps, vs2, start, nbr, ed->n_total, k, pi, meta, mtab, body);
         else gather<GRID, false, true, true>(S, xps, vs2, start, endt, lst, stage, clean, k, pi, body);
     } else if (NBR) {
         gather_row(S, xps, vs2, start, endt, nbr + (size_t)i * kNbrMax, ncnt[i], k, pi, body);
@@ -1812,7 +1909,7 @@ inline uint32_t gather_grid(uint32_t cap, int block)
             if (m_) KER<true, false, false, true, true><<<g_, b_, ldsr_, st>>>(__VA_ARGS__);  \
             else KER<false, false, false, true, true><<<g_, b_, ldsr_, st>>>(__VA_ARGS__);    \
         } else if (s.grid) {                                                                  \
-            const size_t ldsg_ = lds_ + sizeof(float4) * kStageBuf * (size_t)(b_ / 64);       \
+            const size_t ldsg_ = lds_ + sizeof(uint32_t) * gather_wave_words(true, false, false) * (size_t)(b_ / 64); \
             if (m_ && v_.list) KER<true, false, true, false, true><<<g_, b_, ldsg_, st>>>(__VA_ARGS__);  \
             else if (m_) KER<true, false, false, false, true><<<g_, b_, ldsg_, st>>>(__VA_ARGS__);       \
             else if (v_.list) KER<false, false, true, false, true><<<g_, b_, ldsg_, st>>>(__VA_ARGS__);  \
@@ -1842,7 +1939,7 @@ hipError_t launch_density(const DevState &s, const LaunchCfg &c, hipStream_t st)
         const int b = c.density.block;
         const uint32_t g = gather_grid(s.capacity, b);
         const size_t lds = (c.density.list ? sizeof(uint32_t) * kListMax * (size_t)b : 0) +
-                           sizeof(float4) * kStageBuf * (size_t)(b / 64);
+                           sizeof(uint32_t) * gather_wave_words(true, true, false) * (size_t)(b / 64);
         if (s.multi && c.density.list)
             k_density<true, false, true, true, true><<<g, b, lds, st>>>(SPH_GATHER_ARGS, s.dens);
         else if (s.multi)
