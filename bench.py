@@ -497,6 +497,20 @@ def cpu_baseline_c0(threads, warmup_steps=5, steps=20):
 LEGS = ["uniform_1m_grid", "uniform_1m_reference_key", "two_blocks_16m", "dam_break_16k"]
 
 
+def run_serve(args):
+    """The serving workload, configured and plain on the same arrivals, then
+    the sustainable rate (--serve-sustained-seconds 0 skips it)."""
+    configured = bench_serve(args.serve_seconds, args.serve_rate, args.serve_slots, 200)
+    plain = None if args.no_serve_plain else bench_serve_plain(args.serve_seconds, args.serve_rate,
+                                                                args.serve_slots, 200)
+    return {
+        "workload": "serving (Poisson bake requests, 8,192-2,097,152 particles, 100-2,000 steps)",
+        "configured": configured, "plain_same_arrivals": plain,
+        "speedup_vs_plain": configured["value"] / plain["value"] if plain and plain["value"] else None,
+        "sustained": None if args.serve_sustained_seconds <= 0 else
+        bench_serve(args.serve_sustained_seconds, args.serve_sustained_rate, args.serve_slots, 200, seed=5)}
+
+
 def run_legs(names, args):
     """The single-scene legs, each on its own handle."""
     legs = {}
@@ -537,6 +551,7 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=64)
     ap.add_argument("--serve-sustained-rate", type=float, default=0.4)
     ap.add_argument("--serve-sustained-seconds", type=float, default=20.0)
+    ap.add_argument("--no-serve-plain", action="store_true", help="skip the plain serving comparison")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -561,6 +576,8 @@ def main():
         out = {"metric": METRIC, "value": None, "unit": "particle-steps/s", "n_gpus": 1,
                "note": "batched headline skipped (--no-batch)",
                "legs": {} if args.no_legs else run_legs(legs_wanted, args)}
+        if not args.no_serve:
+            out["serve"] = run_serve(args)
         print(json.dumps(out), flush=True)
         return 0
     batch = bench_batch(args.steps, args.warmup, args.plain_steps, args.latency_steps,
@@ -595,14 +612,7 @@ def main():
     if not args.no_legs:
         out["legs"] = run_legs(legs_wanted, args)
     if not args.no_serve:
-        configured = bench_serve(args.serve_seconds, args.serve_rate, args.serve_slots, 200)
-        plain = bench_serve_plain(args.serve_seconds, args.serve_rate, args.serve_slots, 200)
-        out["serve"] = {
-            "workload": "serving (Poisson bake requests, 8,192-2,097,152 particles, 100-2,000 steps)",
-            "configured": configured, "plain_same_arrivals": plain,
-            "speedup_vs_plain": configured["value"] / plain["value"] if plain["value"] else None,
-            "sustained": bench_serve(args.serve_sustained_seconds, args.serve_sustained_rate,
-                                     args.serve_slots, 200, seed=5)}
+        out["serve"] = run_serve(args)
     if not args.no_cpu:
         log("[bench] cpu baselines ...")
         # the reference's PSTL `par` loops run serially without TBB (none in

@@ -299,7 +299,8 @@ int Engine::alloc(uint32_t cap, uint32_t tcap, std::string *err)
     uint32_t nblk = (cap + 1023) / 1024;
     nblk = std::max<uint32_t>(1, std::min<uint32_t>(nblk, 1024));
     d_.radix_blocks = nblk;
-    const uint32_t cblk = (uint32_t)(((uint64_t)tcap + 2047) / 2048);  // counting-sort chunks
+    // counting-sort chunks
+    const uint32_t cblk = (uint32_t)(((uint64_t)tcap + kCountChunkBuckets - 1) / kCountChunkBuckets);
     d_.count_blocks = cblk;
     d_.capacity = cap;
     d_.table_capacity = tcap;
@@ -307,9 +308,12 @@ int Engine::alloc(uint32_t cap, uint32_t tcap, std::string *err)
     if (!rc) rc = check(hipHostMalloc((void **)&ed_h_, sizeof(EngineDesc)), "pinned desc", err);
     if (!rc) rc = dm((void **)&d_ed_, sizeof(EngineDesc), "engine desc");
     if (!rc) rc = dm((void **)&d_sd_, sizeof(SceneDesc) * scene_cap_, "scene table");
-    if (!rc)
-        rc = check(hipHostMalloc((void **)&sd_pinned_, sizeof(SceneDesc) * scene_cap_),
-                   "pinned scene table", err);
+    for (int q = 0; q < kDescRing && !rc; ++q) {
+        rc = check(hipHostMalloc((void **)&desc_ring_[q], sizeof(EngineDesc) + sizeof(SceneDesc) * scene_cap_),
+                   "pinned descriptor staging", err);
+        if (!rc)
+            rc = check(hipEventCreateWithFlags(&desc_ev_[q], hipEventDisableTiming), "descriptor event", err);
+    }
     if (!rc) rc = dm((void **)&d_scene_of_, u4, "scene map");
     if (!rc) rc = dm((void **)&d_.pos, f4, "pos");
     if (!rc) rc = dm((void **)&d_.vel, f4, "vel");
@@ -366,7 +370,10 @@ Engine::~Engine()
     if (colors_buf_) (void)hipFree(colors_buf_);
     if (rpos_) (void)hipFree(rpos_);
     if (ed_h_) (void)hipHostFree(ed_h_);
-    if (sd_pinned_) (void)hipHostFree(sd_pinned_);
+    for (int q = 0; q < kDescRing; ++q) {
+        if (desc_ring_[q]) (void)hipHostFree(desc_ring_[q]);
+        if (desc_ev_[q]) (void)hipEventDestroy(desc_ev_[q]);
+    }
     for (hipEvent_t &e : ev_)
         if (e) (void)hipEventDestroy(e);
     if (stream_) (void)hipStreamDestroy(stream_);
@@ -389,19 +396,24 @@ void Engine::invalidate_graph()
 
 int Engine::push_descs(std::string *err)
 {
-    // The pinned mirrors may still feed an earlier copy: drain the stream first.
-    CK(hipStreamSynchronize(stream_), "sync before descriptor update");
     if (sd_h_.size() > scene_cap_) {
         *err = "scene capacity exceeded";
         return SPH_ERR_INVALID;
     }
+    // the next staging slot, once the copies of its previous use have run
+    // (kDescRing uploads ago: normally long done)
+    const int q = (int)(desc_no_++ % kDescRing);
+    if (desc_live_[q]) CK(hipEventSynchronize(desc_ev_[q]), "descriptor staging");
+    unsigned char *slot = desc_ring_[q];
     const size_t sd_bytes = sizeof(SceneDesc) * sd_h_.size();
-    memcpy(sd_pinned_, sd_h_.data(), sd_bytes);
+    memcpy(slot, ed_h_, sizeof(EngineDesc));
+    if (sd_bytes) memcpy(slot + sizeof(EngineDesc), sd_h_.data(), sd_bytes);
     if (sd_bytes)
-        CK(hipMemcpyAsync(d_sd_, sd_pinned_, sd_bytes, hipMemcpyHostToDevice, stream_),
+        CK(hipMemcpyAsync(d_sd_, slot + sizeof(EngineDesc), sd_bytes, hipMemcpyHostToDevice, stream_),
            "upload scene table");
-    CK(hipMemcpyAsync(d_ed_, ed_h_, sizeof(EngineDesc), hipMemcpyHostToDevice, stream_),
-       "upload engine desc");
+    CK(hipMemcpyAsync(d_ed_, slot, sizeof(EngineDesc), hipMemcpyHostToDevice, stream_), "upload engine desc");
+    CK(hipEventRecord(desc_ev_[q], stream_), "descriptor staging");
+    desc_live_[q] = true;
     return SPH_OK;
 }
 
@@ -471,8 +483,8 @@ int Engine::repack(const std::vector<RepackItem> &items, std::string *err)
     // descriptors of the new layout
     std::vector<SceneDesc> sd(items.size());
     std::vector<sph_params> params(items.size());
-    std::vector<uint32_t> scene_of(n_total);
-    uint32_t k_off = 0;
+    uint32_t k_off = 0, max_n = 0;
+    bool host_sources = false;
     for (size_t q = 0; q < items.size(); ++q) {
         const RepackItem &it = items[q];
         fill_scene_desc(it.spec.params, it.spec.n, it.spec.table, grid_, &sd[q]);
@@ -482,13 +494,12 @@ int Engine::repack(const std::vector<RepackItem> &items, std::string *err)
         sd[q].dt = dt_cached_ < 0.0f ? 0.0f : dt_cached_;
         k_off += it.spec.table;
         params[q] = it.spec.params;
-        for (uint32_t i = 0; i < it.spec.n; ++i) scene_of[new_off[q] + i] = (uint32_t)q;
+        max_n = std::max(max_n, it.spec.n);
+        host_sources = host_sources || (it.old_scene < 0 && !it.gen);
     }
-    if (n_total)
-        CK(hipMemcpyAsync(d_scene_of_, scene_of.data(), sizeof(uint32_t) * n_total,
-                          hipMemcpyHostToDevice, stream_), "repack scene map");
-    // host sources (caller buffers, scene_of) must stay alive until the copies end
-    CK(hipStreamSynchronize(stream_), "repack sync");
+    // caller buffers must stay alive until their copies end; device-only
+    // repacks (moves, generated scenes) leave the stream running
+    if (host_sources) CK(hipStreamSynchronize(stream_), "repack sync");
     std::vector<uint8_t> invalid(items.size(), 0);
     for (size_t q = 0; q < items.size(); ++q)
         if (items[q].old_scene >= 0) invalid[q] = invalid_[items[q].old_scene];
@@ -501,6 +512,8 @@ int Engine::repack(const std::vector<RepackItem> &items, std::string *err)
     work_key_layout(ed_h_->n_scenes, ed_h_);
     int rc = push_descs(err);
     if (rc) return rc;
+    // the slot -> scene map of the new layout, from the scene table just queued
+    CK(launch_scene_map(d_sd_, d_scene_of_, n_scenes(), max_n, stream_), "repack scene map");
     work_valid_ = false;
     have_tables_ = false;
     for (uint32_t q = 0; q < n_scenes(); ++q) CK(refresh_render(q), "repack render positions");
@@ -1069,14 +1082,16 @@ int Engine::download(uint32_t s, float *pos4, float *vel4, float *dens2, std::st
             memset(dens2, 0, sizeof(float) * 2 * d.n);
             return SPH_OK;
         }
-        float2 *tmp = nullptr;
-        CK(hipMalloc((void **)&tmp, sizeof(float2) * d_.capacity), "density staging");
+        // particle order through the repack staging where it exists (its
+        // capacity float4s hold the capacity float2s), else a buffer of its own
+        float2 *tmp = reinterpret_cast<float2 *>(tmp_pos_);
+        if (!tmp) CK(hipMalloc((void **)&tmp, sizeof(float2) * d_.capacity), "density staging");
         hipError_t e1 = launch_unpermute_dens(d_, tmp, stream_);
         hipError_t e2 = e1 == hipSuccess ? hipStreamSynchronize(stream_) : e1;
         hipError_t e3 = e2 == hipSuccess ? hipMemcpy(dens2, tmp + d.p_off, sizeof(float2) * d.n,
                                                      hipMemcpyDeviceToHost)
                                          : e2;
-        (void)hipFree(tmp);
+        if (tmp != reinterpret_cast<float2 *>(tmp_pos_)) (void)hipFree(tmp);
         CK(e3, "download densities");
     }
     return SPH_OK;

@@ -160,8 +160,16 @@ class Engine {
     LaunchCfg cfg_{};
     std::vector<SceneDesc> sd_h_;
     std::vector<sph_params> params_;
-    EngineDesc *ed_h_ = nullptr;  // pinned mirror
-    SceneDesc *sd_pinned_ = nullptr;
+    EngineDesc *ed_h_ = nullptr;  // host copy of the engine descriptor
+    // Descriptor uploads go through a ring of pinned staging slots (engine
+    // descriptor, then the scene table), each reused only after the copies
+    // from its previous use have run (its event), so an upload never waits
+    // for the stream to drain.
+    static constexpr int kDescRing = 4;
+    unsigned char *desc_ring_[kDescRing] = {};
+    hipEvent_t desc_ev_[kDescRing] = {};
+    bool desc_live_[kDescRing] = {};
+    uint32_t desc_no_ = 0;
     EngineDesc *d_ed_ = nullptr;
     SceneDesc *d_sd_ = nullptr;
     uint32_t *d_scene_of_ = nullptr;

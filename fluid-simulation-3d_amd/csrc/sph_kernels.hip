@@ -51,6 +51,48 @@ constexpr int kSortWaves = kSortBlock / 64;
 __device__ __forceinline__ uint32_t grid_stride() { return gridDim.x * blockDim.x; }
 __device__ __forceinline__ uint32_t global_id() { return blockIdx.x * blockDim.x + threadIdx.x; }
 
+// Scene q's descriptor for a wave-uniform q, read through the constant
+// address space: the table does not change while a kernel runs, so the reads
+// are scalar loads into scalar registers wherever they are placed.
+__device__ __forceinline__ SceneDesc scene_uniform(const SceneDesc *sd, uint32_t q)
+{
+#if __HIP_DEVICE_COMPILE__
+    typedef const __attribute__((address_space(4))) uint32_t ConstWord;
+    const ConstWord *w = (const ConstWord *)(sd + q);
+    SceneDesc d;
+    uint32_t *o = reinterpret_cast<uint32_t *>(&d);
+#pragma unroll
+    for (uint32_t j = 0; j < sizeof(SceneDesc) / 4; ++j) o[j] = w[j];
+    return d;
+#else
+    return sd[q];  // (host pass: never called)
+#endif
+}
+
+// Runs body(S) with slot k's scene.  MULTI: the wave takes its scenes one at
+// a time (one pass unless it spans a scene boundary), the lanes of that scene
+// active and its descriptor read with scalar loads into scalar registers.
+// The loop runs on a wave-uniform mask, and a lane's membership is read back
+// from a ballot, so the compiler keeps the body inside the loop with the
+// uniform scene index (rather than substituting the per-lane one).
+template <bool MULTI, typename Body>
+__device__ __forceinline__ void run_scene(const SceneDesc *sd, const uint32_t *scene_of, uint32_t k,
+                                          Body &body)
+{
+    if (!MULTI) {
+        body(sd[0]);
+        return;
+    }
+    const uint32_t q = scene_of[k], lane = threadIdx.x & 63u;
+    uint64_t pending = __ballot(1);
+    while (pending) {
+        const uint32_t qq = (uint32_t)__builtin_amdgcn_readlane((int)q, (int)__builtin_ctzll(pending));
+        const uint64_t mine = __ballot(q == qq);
+        if ((mine >> lane) & 1u) body(scene_uniform(sd, qq));
+        pending &= ~mine;
+    }
+}
+
 template <bool MULTI>
 __device__ __forceinline__ const SceneDesc &scene_of_slot(const SceneDesc *sd,
                                                           const uint32_t *scene_of, uint32_t k)
@@ -351,17 +393,23 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_scatter(
 //                    bucket's slots with a smaller index; gather into sorted
 //                    order; clear the counts for the next update
 // ---------------------------------------------------------------------------
-constexpr uint32_t kCountPer = 8;                          // buckets per work-item
-constexpr uint32_t kCountChunk = kSortBlock * kCountPer;   // buckets per block
+constexpr uint32_t kCountChunk = kCountChunkBuckets;      // buckets per block
+constexpr uint32_t kCountPer = kCountChunk / kSortBlock;   // buckets per work-item
+static_assert(kCountPer % 4 == 0 && kCountPer * kSortBlock == kCountChunk, "count chunk");
 
 __device__ __forceinline__ void load_counts(const uint32_t *__restrict__ count, uint32_t b0,
                                             uint32_t T, uint32_t c[kCountPer])
 {
     if (b0 + kCountPer <= T) {
         const uint4 *q = reinterpret_cast<const uint4 *>(count + b0);
-        const uint4 u = q[0], v = q[1];
-        c[0] = u.x; c[1] = u.y; c[2] = u.z; c[3] = u.w;
-        c[4] = v.x; c[5] = v.y; c[6] = v.z; c[7] = v.w;
+#pragma unroll
+        for (uint32_t j = 0; j < kCountPer / 4; ++j) {
+            const uint4 u = q[j];
+            c[4 * j] = u.x;
+            c[4 * j + 1] = u.y;
+            c[4 * j + 2] = u.z;
+            c[4 * j + 3] = u.w;
+        }
     } else {
 #pragma unroll
         for (uint32_t j = 0; j < kCountPer; ++j) c[j] = b0 + j < T ? count[b0 + j] : 0u;
@@ -408,10 +456,15 @@ __global__ __launch_bounds__(kSortBlock) void k_count_scan(const EngineDesc *__r
     __shared__ uint32_t chunk_s, base_s;
     const uint32_t T = ed->table_total, tid = threadIdx.x, nb = gridDim.x;
     const uint32_t n = ed->n_total, spin_max = ed->scan_spin;
+    // the grid covers the table's capacity; only the chunks of the T buckets
+    // in use scan (at least one, which writes bounds[T] and the scene flags),
+    // and only their workgroups take tickets (tickets 0 .. na-1)
+    const uint32_t na = min(nb, max(1u, (T + kCountChunk - 1) / kCountChunk));
+    if (blockIdx.x >= na) return;
     if (tid == 0) chunk_s = atomicAdd(reinterpret_cast<uint32_t *>(chain + nb), 1u);
     __syncthreads();
     const uint32_t c = chunk_s;
-    if (c >= nb) return;  // never taken while the counter starts at zero
+    if (c >= na) return;  // never: na workgroups take na tickets
     const uint32_t b0 = c * kCountChunk + tid * kCountPer;
     uint32_t cnt[kCountPer];
     load_counts(count, b0, T, cnt);
@@ -459,8 +512,9 @@ __global__ __launch_bounds__(kSortBlock) void k_count_scan(const EngineDesc *__r
             o[j] = min(p, n);
             p += cnt[j];
         }
-        q[0] = make_uint4(o[0], o[1], o[2], o[3]);
-        q[1] = make_uint4(o[4], o[5], o[6], o[7]);
+#pragma unroll
+        for (uint32_t j = 0; j < kCountPer / 4; ++j)
+            q[j] = make_uint4(o[4 * j], o[4 * j + 1], o[4 * j + 2], o[4 * j + 3]);
     } else {
 #pragma unroll
         for (uint32_t j = 0; j < kCountPer; ++j) {
@@ -468,7 +522,7 @@ __global__ __launch_bounds__(kSortBlock) void k_count_scan(const EngineDesc *__r
             p += cnt[j];
         }
     }
-    if (c == nb - 1) {
+    if (c == na - 1) {
         if (tid == 0) bounds[T] = min(base_s + total, n);
         for (uint32_t q = tid; q < ed->n_scenes; q += kSortBlock) sflags[q] = count[T + q];
     }
@@ -615,6 +669,16 @@ __global__ void k_unpermute_state(const float4 *__restrict__ pos, const float4 *
 __global__ void k_iota(uint32_t *__restrict__ a, uint32_t n, uint32_t first)
 {
     for (uint32_t k = global_id(); k < n; k += grid_stride()) a[k] = first + k;
+}
+
+// scene_of[p_off + i] = s for every scene s (blockIdx.y) and i < n, from the
+// scene table on the device
+__global__ void k_scene_map(const SceneDesc *__restrict__ sd, uint32_t *__restrict__ scene_of)
+{
+    const SceneDesc &S = sd[blockIdx.y];
+    const uint32_t n = S.n, off = S.p_off;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        scene_of[off + i] = blockIdx.y;
 }
 
 // ---------------------------------------------------------------------------
@@ -1273,15 +1337,35 @@ __device__ __forceinline__ uint32_t walk(const SceneDesc &S, const float4 *__res
 constexpr uint32_t kListMax = 56;
 
 // GRID: work-item i handles sorted slot i (cells in grid order, already
-//   spatially coherent), with the XCD-contiguous workgroup numbering.
-template <bool WORK, bool GRID>
-__device__ __forceinline__ uint32_t gather_index()
+//   spatially coherent), with the XCD-aware workgroup numbering.
+// The grid covers the engine's capacity (a multiple of 8 workgroups); the
+// numbering spreads the n live slots' workgroups over all 8 XCDs (block b
+// runs on XCD b % 8), and the workgroups past them get whole blocks of
+// indices >= n (block numbers from `live` on, below 2^32 / blockDim.x since
+// the grid's capacity is < 2^31).  One scene: XCD x takes the contiguous
+// live blocks [x * per, (x + 1) * per), so each XCD's L2 holds one slab of
+// the grid.  Several scenes (MULTI): chunks of kXcdChunk consecutive blocks
+// are dealt to the XCDs in turn (the blocks past the last whole chunks
+// contiguously), so every XCD gets a sample of every part of the layout --
+// scenes differ in cost per particle (a compressed pool against a falling
+// block), and one XCD per slab would leave the others idle while it ends.
+constexpr uint32_t kXcdChunk = 64;
+template <bool WORK, bool GRID, bool MULTI = false>
+__device__ __forceinline__ uint32_t gather_index(uint32_t n)
 {
     if (!WORK && !GRID) return blockIdx.x * blockDim.x + threadIdx.x;
-    // workgroups are dealt round-robin to the 8 XCDs (block b -> XCD b % 8);
-    // the grid is a multiple of 8
-    const uint32_t G = gridDim.x, b = blockIdx.x;
-    const uint32_t lb = (b & 7u) * (G >> 3) + (b >> 3);
+    const uint32_t live = (n + blockDim.x - 1) / blockDim.x;
+    const uint32_t per = (live + 7u) >> 3, b = blockIdx.x, x = b & 7u, j = b >> 3;
+    uint32_t lb;
+    if (j >= per) {
+        lb = live + j;
+    } else if (!MULTI) {
+        lb = x * per + j;
+    } else {
+        const uint32_t C = max(1u, min(kXcdChunk, per >> 3));
+        const uint32_t full = per / C * C;
+        lb = j < full ? ((j / C) * 8u + x) * C + j % C : 8u * full + x * (per - full) + (j - full);
+    }
     return lb * blockDim.x + threadIdx.x;
 }
 
@@ -1358,11 +1442,10 @@ __host__ __device__ constexpr uint32_t gather_wave_words(bool grid, bool nbr, bo
 
 #define SPH_GATHER_PROLOGUE()                                                       \
     extern __shared__ uint32_t lds_list[];                                          \
-    const uint32_t i = gather_index<WORK, GRID>();                                  \
+    const uint32_t i = gather_index<WORK, GRID, MULTI>(ed->n_total);               \
     if (i >= ed->n_total) return;                                                   \
     const uint32_t g = WORK ? work[i] : svals[i];                                   \
     const uint32_t k = WORK ? rank[g] : i;                                          \
-    const SceneDesc &S = scene_of_slot<MULTI>(sd, scene_of, k);                     \
     const float4 pi = xps[k];                                                       \
     uint32_t *lst = lds_list + threadIdx.x;                                         \
     /* GRID: each wave's LDS follows the lists (gather_wave_words words).    */  \
@@ -1375,6 +1458,9 @@ __host__ __device__ constexpr uint32_t gather_wave_words(bool grid, bool nbr, bo
     float4 *stage = (GRID && !CONSUMER) ? reinterpret_cast<float4 *>(wave_lds) : nullptr; \
     uint32_t *mwords = wave_lds + 4 * kStageBuf + (threadIdx.x & 63u);            \
     uint32_t *mtab = wave_lds + (threadIdx.x & 63u);                               \
+    /* the rest of the kernel, for scene S (closed by SPH_GATHER_EPILOGUE,   */  \
+    /* which runs it through run_scene)                                      */  \
+    auto scene_body_ = [&](const SceneDesc &S) __attribute__((always_inline)) {    \
     /* grid key: no particle of this scene has its cell outside the grid's box */ \
     const bool clean = GRID && sflags[S.index] == 0u;                              \
     (void)mtab;                                                                     \
@@ -1382,6 +1468,11 @@ __host__ __device__ constexpr uint32_t gather_wave_words(bool grid, bool nbr, bo
     (void)clean;                                                                    \
     (void)g;                                                                        \
     (void)lst
+
+#define SPH_GATHER_EPILOGUE()                                                       \
+    }                                                                               \
+    ;                                                                               \
+    run_scene<MULTI>(sd, scene_of, k, scene_body_)
 
 #define SPH_GATHER_PARAMS                                                                     \
     const EngineDesc *__restrict__ ed, const SceneDesc *__restrict__ sd,                      \
@@ -1488,6 +1579,7 @@ __global__ __launch_bounds__(256, MULTI ? 6 : 8) void k_density(SPH_GATHER_PARAM
     // pressures of cc:371-372 / 407-408, computed once per particle
     // the pressure pass divides by these densities: their reciprocals ride along
     dens[k] = make_float4(rho, rhon, reciprocal_or_zero(rho), reciprocal_or_zero(rhon));
+    SPH_GATHER_EPILOGUE();
 }
 
 // Density pass of the neighbour-row mode.  Each work-item stages its row in
@@ -1502,7 +1594,7 @@ __global__ __launch_bounds__(256) void k_density_rows(SPH_GATHER_PARAMS, float4 
 {
     extern __shared__ uint32_t lds_rows[];  // [blockDim.x][kRowPitch], then counts[blockDim.x]
     const uint32_t B = blockDim.x, t = threadIdx.x;
-    const uint32_t i = gather_index<true, false>();
+    const uint32_t i = gather_index<true, false>(ed->n_total);
     const uint32_t n = ed->n_total;
     uint32_t *my = lds_rows + t * kRowPitch;
     uint32_t *cnt = lds_rows + B * kRowPitch;
@@ -1616,6 +1708,7 @@ __global__ __launch_bounds__(256) void k_pressure(SPH_GATHER_PARAMS, const float
     v.y = v.y + (fy / rho) * dt;
     v.z = v.z + (fz / rho) * dt;
     vs2[k] = v;
+    SPH_GATHER_EPILOGUE();
 }
 
 // Writes the new state at the sorted slot (pos, vel, ids: the next resident
@@ -1687,6 +1780,7 @@ __global__ __launch_bounds__(256, MULTI ? 4 : 5) void k_viscosity_integrate(SPH_
     }
     // stage A + B1 of the next update, from the new state
     record_bucket(*ed, S, predict_particle(S, x, v), true, k, keys, ticks, count, wkeys);
+    SPH_GATHER_EPILOGUE();
 }
 
 // ---------------------------------------------------------------------------
@@ -1995,6 +2089,15 @@ hipError_t launch_unpermute_state(const DevState &s, uint32_t off, uint32_t n, f
 hipError_t launch_iota(uint32_t *a, uint32_t n, uint32_t first, hipStream_t st)
 {
     k_iota<<<grid_for(n, 256), 256, 0, st>>>(a, n, first);
+    SPH_LAUNCH_CHECK();
+}
+
+hipError_t launch_scene_map(const SceneDesc *sd, uint32_t *scene_of, uint32_t n_scenes, uint32_t max_n,
+                            hipStream_t st)
+{
+    if (n_scenes == 0) return hipSuccess;
+    const uint32_t bx = grid_for(max_n, 256) < 1024u ? grid_for(max_n, 256) : 1024u;
+    k_scene_map<<<dim3(bx, n_scenes), 256, 0, st>>>(sd, scene_of);
     SPH_LAUNCH_CHECK();
 }
 

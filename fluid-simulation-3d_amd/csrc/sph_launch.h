@@ -10,6 +10,10 @@
 
 namespace sph {
 
+// buckets per workgroup of the one-pass count scan: a chain status word and a
+// ticket (one atomic on a single counter) per chunk
+constexpr uint32_t kCountChunkBuckets = 4096;
+
 struct DevState {
     // engine descriptors
     const EngineDesc *ed;
@@ -129,6 +133,10 @@ hipError_t launch_unpermute_state(const DevState &s, uint32_t off, uint32_t n, f
                                   float4 *dst_vel, hipStream_t st);
 // a[k] = first + k for k < n.
 hipError_t launch_iota(uint32_t *a, uint32_t n, uint32_t first, hipStream_t st);
+// scene_of of every slot of the first n_scenes scenes, from the device scene
+// table (max_n: the largest scene's particle count)
+hipError_t launch_scene_map(const SceneDesc *sd, uint32_t *scene_of, uint32_t n_scenes, uint32_t max_n,
+                            hipStream_t st);
 // a[k] = 0 for k < n, by a fill kernel (no memset node when captured).
 hipError_t launch_clear_words(uint32_t *a, uint32_t n, hipStream_t st);
 // Neighbour lists for parity checks (particle order, walk order, self included).

@@ -8,7 +8,7 @@
 // with the handle's buffers replaced by regions of `scratch`.
 //
 // `scratch` is one file-scope device array per GPU, held until the process
-// ends: SPH_NOALLOC_SCRATCH_BYTES (24,647,424, about 23.5 MiB) for up to kNoAllocMaxParticles
+// ends: SPH_NOALLOC_SCRATCH_BYTES (24,645,376, about 23.5 MiB) for up to kNoAllocMaxParticles
 // particles and kNoAllocMaxBuckets buckets.  Its device address is looked up
 // once per device with hipGetSymbolAddress (the array's name in host code is
 // a host-side stand-in, not a device address) and kept per device.  Every call
@@ -34,7 +34,7 @@ constexpr uint32_t kNoAllocMaxParticles = SPH_NOALLOC_MAX_PARTICLES;
 // the cell-grid key sizes a scene of n <= 2^16 particles to at most
 // max(8n, 2^20) = 2^20 buckets (grid_table_size)
 constexpr uint32_t kNoAllocMaxBuckets = 1u << 20;
-constexpr uint32_t kCountChunk = 2048;  // buckets per workgroup of the count scan
+constexpr uint32_t kCountChunk = sph::kCountChunkBuckets;  // buckets per workgroup of the count scan
 
 constexpr size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
@@ -49,7 +49,7 @@ struct Layout {
 constexpr Layout make_layout()
 {
     const size_t N = kNoAllocMaxParticles, T = kNoAllocMaxBuckets;
-    const size_t chain_words = T / kCountChunk + 1;
+    const size_t chain_words = (T + kCountChunk - 1) / kCountChunk + 1;
     Layout L{};
     size_t o = 0;
     auto take = [&o](size_t bytes) {

@@ -239,7 +239,7 @@ int sph_count_nonfinite(sph_handle h, uint64_t *count);
  * Every call on one GPU shares that GPU's array: calls that overlap on one GPU
  * (from several threads, or on streams not ordered with each other) race. */
 #define SPH_NOALLOC_MAX_PARTICLES 65536u
-#define SPH_NOALLOC_SCRATCH_BYTES 24647424u
+#define SPH_NOALLOC_SCRATCH_BYTES 24645376u
 int sph_update_noalloc(const sph_params *p, uint32_t n, float dt, const float *pos4_in,
                        const float *vel4_in, float *pos4_out, float *vel4_out, float *dens2_out,
                        void *stream, int wait);

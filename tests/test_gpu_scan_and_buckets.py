@@ -42,7 +42,7 @@ def _expect_exact(sim, sc, dt, what):
 
 
 def test_scan_give_up_is_reported_and_recoverable():
-    # 2^24 buckets: 8192 chunks of the count scan, so with a zero bound some
+    # 2^24 buckets: 4096 chunks of the count scan, so with a zero bound some
     # workgroup finds a predecessor's chunk unpublished
     sc = W.uniform_lower(n=65536, table=1 << 24, box=(8.0, 2.0, 8.0))
     dt = _sub_dt()
