@@ -1349,7 +1349,7 @@ constexpr uint32_t kListMax = 56;
 // contiguously), so every XCD gets a sample of every part of the layout --
 // scenes differ in cost per particle (a compressed pool against a falling
 // block), and one XCD per slab would leave the others idle while it ends.
-constexpr uint32_t kXcdChunk = 64;
+constexpr uint32_t kXcdChunk = 128;
 template <bool WORK, bool GRID, bool MULTI = false>
 __device__ __forceinline__ uint32_t gather_index(uint32_t n)
 {
