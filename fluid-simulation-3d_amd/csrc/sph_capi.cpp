@@ -400,7 +400,7 @@ int sph_autotune(sph_handle h, float dt, int updates, double ms[3])
 }
 
 namespace {
-int get_variants(sph::Engine *e, int32_t v[13])
+int get_variants(sph::Engine *e, int32_t v[14])
 {
     if (!v) return fail(SPH_ERR_INVALID, "null output");
     const sph::LaunchCfg &c = e->launch_cfg();
@@ -412,10 +412,11 @@ int get_variants(sph::Engine *e, int32_t v[13])
         v[4 * q + 3] = g[q].rows;
     }
     v[12] = c.nbr;
+    v[13] = c.fused;
     return SPH_OK;
 }
 
-int set_variants(sph::Engine *e, const int32_t v[13])
+int set_variants(sph::Engine *e, const int32_t v[14])
 {
     if (!v) return fail(SPH_ERR_INVALID, "null input");
     sph::LaunchCfg c = e->launch_cfg();
@@ -432,18 +433,21 @@ int set_variants(sph::Engine *e, const int32_t v[13])
     }
     if ((c.pressure.rows || c.viscosity.rows) && !c.nbr)
         return fail(SPH_ERR_INVALID, "reading neighbour rows needs the row mode (v[12])");
+    c.fused = v[13] != 0;
+    if (c.fused && !(c.nbr && c.density.rows && c.pressure.rows && c.viscosity.rows))
+        return fail(SPH_ERR_INVALID, "the fused update (v[13]) needs the row mode in all three kernels");
     e->set_launch_cfg(c);
     return SPH_OK;
 }
 }  // namespace
 
-int sph_get_variants(sph_handle h, int32_t v[13])
+int sph_get_variants(sph_handle h, int32_t v[14])
 {
     SPH_NEED(h);
     return get_variants(h->eng, v);
 }
 
-int sph_set_variants(sph_handle h, const int32_t v[13])
+int sph_set_variants(sph_handle h, const int32_t v[14])
 {
     SPH_NEED(h);
     return set_variants(h->eng, v);
@@ -553,13 +557,13 @@ int sph_batch_profile_end(sph_batch b, double mean_ms[6], uint32_t *updates)
     SPH_BATCH_CALL(b->eng->profile_end(mean_ms, updates, &err))
 }
 
-int sph_batch_get_variants(sph_batch b, int32_t v[13])
+int sph_batch_get_variants(sph_batch b, int32_t v[14])
 {
     SPH_NEED(b);
     return get_variants(b->eng, v);
 }
 
-int sph_batch_set_variants(sph_batch b, const int32_t v[13])
+int sph_batch_set_variants(sph_batch b, const int32_t v[14])
 {
     SPH_NEED(b);
     return set_variants(b->eng, v);

@@ -255,6 +255,10 @@ int Engine::create(const std::vector<SceneSpec> &scenes, uint32_t capacity,
     e->grid_ = grid;
     e->d_.grid = grid;
     e->cfg_ = default_launch_cfg(grid);
+    {
+        int dev = 0;
+        if (hipGetDevice(&dev) == hipSuccess) e->d_.fused_grid = fused_update_grid(e->d_.multi, dev);
+    }
     rc = e->check(hipMemcpy(e->d_scene_of_, scene_of.data(), sizeof(uint32_t) * capacity,
                             hipMemcpyHostToDevice),
                   "upload scene map", err);
@@ -344,6 +348,8 @@ int Engine::alloc(uint32_t cap, uint32_t tcap, std::string *err)
     if (!rc) rc = dm((void **)&d_.sflags, sizeof(uint32_t) * scene_cap_, "scene flags");
     if (!rc) rc = dm((void **)&d_.scan_err, sizeof(uint32_t), "scan error flag");
     if (!rc) rc = check(hipMemset(d_.scan_err, 0, sizeof(uint32_t)), "scan error flag", err);
+    if (!rc) rc = dm((void **)&d_.fbar, sizeof(uint32_t) * kFusedBarWords, "grid barrier");
+    if (!rc) rc = check(hipMemset(d_.fbar, 0, sizeof(uint32_t) * kFusedBarWords), "grid barrier", err);
     // wrapped until a predict pass says otherwise
     if (!rc) rc = check(hipMemset(d_.sflags, 0xff, sizeof(uint32_t) * scene_cap_), "scene flags", err);
     d_.ed = d_ed_;
@@ -362,7 +368,7 @@ Engine::~Engine()
                     d_.ids,      d_.ps,       d_.order,    d_.sids,
                     d_.hist,     d_.bounds,   d_.count,    d_.chain,
                     d_.wkeys[0], d_.wkeys[1], d_.wvals[0], d_.wvals[1], d_.rank,
-                    d_.nbr,      d_.ncnt,     d_.sflags,   d_.scan_err};
+                    d_.nbr,      d_.ncnt,     d_.sflags,   d_.scan_err, d_.fbar};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (tmp_pos_) (void)hipFree(tmp_pos_);
@@ -699,6 +705,11 @@ hipError_t Engine::enqueue_update(hipEvent_t *ev, bool rebuild_work, bool fresh,
         d_.work = d_.wvals[wb];
         work_valid_ = true;
     }
+    if (fused_ok(rebuild_work, fresh) && !ev) {
+        // one persistent launch for the whole steady update
+        LK(launch_fused_update(d_, stream_));
+        return hipSuccess;
+    }
     LK(launch_bucket_sort(d_, cfg_, stream_));
     if (ev) LK(hipEventRecord(ev[2], stream_));
     LK(launch_density(d_, cfg_, stream_));
@@ -862,16 +873,54 @@ int Engine::autotune(float dt, int updates, double ms_out[3], std::string *err)
             best[q] = walk_ms[q];
         }
     }
-    if (!rc) rc = check(hipStreamSynchronize(stream_), "autotune sync", err);
-    (void)hipFree(snap);
-    timing_ = keep_timing;
     LaunchCfg chosen = keep;
     if (!rc) {
         chosen.density = pick[0];
         chosen.pressure = pick[1];
         chosen.viscosity = pick[2];
         chosen.nbr = use_nbr;
+        chosen.fused = false;
     }
+    // The fused update (one persistent launch per steady update) against the
+    // separate launches of the chosen variants: `updates` steady updates
+    // each, timed whole, two runs of each interleaved, the faster kept.
+    cfg_ = chosen;
+    cfg_.fused = true;
+    const bool try_fused = !rc && fused_ok(false, false);
+    cfg_ = chosen;
+    if (try_fused) {
+        hipEvent_t t0 = nullptr, t1 = nullptr;
+        rc = check(hipEventCreate(&t0), "autotune timer", err);
+        if (!rc) rc = check(hipEventCreate(&t1), "autotune timer", err);
+        double best_ms[2] = {1e30, 1e30};
+        for (int trial = 0; trial < 4 && !rc; ++trial) {
+            const int f = trial & 1;
+            cfg_ = chosen;
+            cfg_.fused = f != 0;
+            invalidate_graph();
+            // the first update after the restore is fresh (predict runs); the
+            // next one records the buckets, so the timed ones are steady
+            rc = update(dt, err);
+            if (!rc) rc = update(dt, err);
+            if (!rc) rc = check(hipEventRecord(t0, stream_), "autotune timer", err);
+            for (int u = 0; u < updates && !rc; ++u) rc = update(dt, err);
+            if (!rc) rc = check(hipEventRecord(t1, stream_), "autotune timer", err);
+            if (!rc) rc = check(hipEventSynchronize(t1), "autotune timer", err);
+            float ms = 0.0f;
+            if (!rc) rc = check(hipEventElapsedTime(&ms, t0, t1), "autotune timer", err);
+            if (!rc) rc = device_status(-1, err);
+            if (!rc) best_ms[f] = std::min(best_ms[f], (double)ms);
+            if (!rc) rc = check(restore(), "autotune restore", err);
+            for (uint32_t q = 0; q < n_scenes() && !rc; ++q)
+                rc = check(refresh_render(q), "autotune restore", err);
+        }
+        if (t0) (void)hipEventDestroy(t0);
+        if (t1) (void)hipEventDestroy(t1);
+        if (!rc) chosen.fused = best_ms[1] < best_ms[0];
+    }
+    if (!rc) rc = check(hipStreamSynchronize(stream_), "autotune sync", err);
+    (void)hipFree(snap);
+    timing_ = keep_timing;
     cfg_ = chosen;
     invalidate_graph();
     work_valid_ = false;
@@ -997,14 +1046,17 @@ int Engine::device_status(int s, std::string *err)
     if (flag) {
         for (uint8_t &v : invalid_) v = 1;
         CK(hipMemset(d_.scan_err, 0, sizeof(uint32_t)), "clear device status");
+        // a fused update's barrier that gave up leaves its count behind
+        CK(hipMemset(d_.fbar, 0, sizeof(uint32_t) * kFusedBarWords), "clear device status");
     }
     bool bad = false;
     if (s >= 0) bad = (size_t)s < invalid_.size() && invalid_[s];
     else
         for (uint8_t v : invalid_) bad = bad || v;
     if (!bad) return SPH_OK;
-    *err = "the count scan's bounded look-back wait gave up during an update (device busy "
-           "beyond its bound): the state is invalid until new state is uploaded or generated";
+    *err = "a bounded wait of an update gave up (the count scan's look-back, or a grid barrier of "
+           "the fused update; device busy beyond its bound): the state is invalid until new state "
+           "is uploaded or generated";
     return SPH_ERR_DEVICE;
 }
 

@@ -125,6 +125,13 @@ class Engine {
     // recomputed by the next update.
     int autotune(float dt, int updates, double ms_out[3], std::string *err);
     void invalidate_graph();
+    // the fused update applies: chosen, grid key with neighbour masks in all
+    // three kernels, a steady update, and a launch the device holds resident
+    bool fused_ok(bool rebuild_work, bool fresh) const
+    {
+        return cfg_.fused && grid_ && cfg_.nbr && cfg_.density.rows && cfg_.pressure.rows &&
+               cfg_.viscosity.rows && !rebuild_work && !fresh && d_.fused_grid != 0;
+    }
     int set_use_graph(bool on) { use_graph_ = on; invalidate_graph(); return 0; }
     // Mean duration (ms) of each gather kernel over the last timed update.
     uint64_t device_bytes() const { return device_bytes_; }

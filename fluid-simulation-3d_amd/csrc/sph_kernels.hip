@@ -445,26 +445,24 @@ __device__ __forceinline__ void chain_store(uint64_t *p, uint64_t v)
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__global__ __launch_bounds__(kSortBlock) void k_count_scan(const EngineDesc *__restrict__ ed,
-                                                            const uint32_t *__restrict__ count,
-                                                            uint64_t *__restrict__ chain,
-                                                            uint32_t *__restrict__ bounds,
-                                                            uint32_t *__restrict__ sflags,
-                                                            uint32_t *__restrict__ scan_err)
+// Chunks of the count scan holding the T buckets in use (at least one,
+// which writes bounds[T] and the scene flags); nb = the chunks the table's
+// capacity needs.
+__device__ __forceinline__ uint32_t scan_chunks_live(const EngineDesc *ed, uint32_t nb)
+{
+    return min(nb, max(1u, (ed->table_total + kCountChunk - 1) / kCountChunk));
+}
+
+// Chunk c (< na) of the count scan, by one workgroup of kSortBlock.
+__device__ __forceinline__ void scan_chunk(uint32_t c, uint32_t na, const EngineDesc *__restrict__ ed,
+                                           const uint32_t *__restrict__ count, uint64_t *__restrict__ chain,
+                                           uint32_t *__restrict__ bounds, uint32_t *__restrict__ sflags,
+                                           uint32_t *__restrict__ scan_err)
 {
     __shared__ uint32_t wsum[kSortWaves];
-    __shared__ uint32_t chunk_s, base_s;
-    const uint32_t T = ed->table_total, tid = threadIdx.x, nb = gridDim.x;
+    __shared__ uint32_t base_s;
+    const uint32_t T = ed->table_total, tid = threadIdx.x;
     const uint32_t n = ed->n_total, spin_max = ed->scan_spin;
-    // the grid covers the table's capacity; only the chunks of the T buckets
-    // in use scan (at least one, which writes bounds[T] and the scene flags),
-    // and only their workgroups take tickets (tickets 0 .. na-1)
-    const uint32_t na = min(nb, max(1u, (T + kCountChunk - 1) / kCountChunk));
-    if (blockIdx.x >= na) return;
-    if (tid == 0) chunk_s = atomicAdd(reinterpret_cast<uint32_t *>(chain + nb), 1u);
-    __syncthreads();
-    const uint32_t c = chunk_s;
-    if (c >= na) return;  // never: na workgroups take na tickets
     const uint32_t b0 = c * kCountChunk + tid * kCountPer;
     uint32_t cnt[kCountPer];
     load_counts(count, b0, T, cnt);
@@ -528,18 +526,47 @@ __global__ __launch_bounds__(kSortBlock) void k_count_scan(const EngineDesc *__r
     }
 }
 
+__global__ __launch_bounds__(kSortBlock) void k_count_scan(const EngineDesc *__restrict__ ed,
+                                                            const uint32_t *__restrict__ count,
+                                                            uint64_t *__restrict__ chain,
+                                                            uint32_t *__restrict__ bounds,
+                                                            uint32_t *__restrict__ sflags,
+                                                            uint32_t *__restrict__ scan_err)
+{
+    __shared__ uint32_t chunk_s;
+    // the grid covers the table's capacity; only the live chunks' workgroups
+    // take tickets (tickets 0 .. na-1)
+    const uint32_t nb = gridDim.x, na = scan_chunks_live(ed, nb);
+    if (blockIdx.x >= na) return;
+    if (threadIdx.x == 0) chunk_s = atomicAdd(reinterpret_cast<uint32_t *>(chain + nb), 1u);
+    __syncthreads();
+    const uint32_t c = chunk_s;
+    if (c >= na) return;  // never: na workgroups take na tickets
+    scan_chunk(c, na, ed, count, chain, bounds, sflags, scan_err);
+}
+
 // order[bounds[key] + ticket] = (resident slot, particle index)
+// (work-items first, first + stride, ... of the grid-stride loop)
+__device__ __forceinline__ void scatter_range(uint32_t first, uint32_t stride, const EngineDesc *__restrict__ ed,
+                                              const uint32_t *__restrict__ keys,
+                                              const uint32_t *__restrict__ ticks,
+                                              const uint32_t *__restrict__ ids,
+                                              const uint32_t *__restrict__ bounds, uint2 *__restrict__ order)
+{
+    const uint32_t n = ed->n_total;
+    for (uint32_t g = first; g < n; g += stride) {
+        const uint32_t slot = bounds[keys[g]] + ticks[g];  // < n: the counts started at zero
+        if (slot < n) order[slot] = make_uint2(g, ids[g]);
+    }
+}
+
 __global__ void k_count_scatter(const EngineDesc *__restrict__ ed,
                                 const uint32_t *__restrict__ keys,
                                 const uint32_t *__restrict__ ticks,
                                 const uint32_t *__restrict__ ids,
                                 const uint32_t *__restrict__ bounds, uint2 *__restrict__ order)
 {
-    const uint32_t n = ed->n_total;
-    for (uint32_t g = global_id(); g < n; g += grid_stride()) {
-        const uint32_t slot = bounds[keys[g]] + ticks[g];  // < n: the counts started at zero
-        if (slot < n) order[slot] = make_uint2(g, ids[g]);
-    }
+    scatter_range(global_id(), grid_stride(), ed, keys, ticks, ids, bounds, order);
 }
 
 // A record's sorted slot is its bucket's first slot plus the number of the
@@ -563,9 +590,10 @@ __global__ void k_count_scatter(const EngineDesc *__restrict__ ed,
 constexpr uint32_t kRankDirect = 64;
 constexpr uint32_t kReorderBlock = 256;
 
+// Workgroup b of nblk (of kReorderBlock work-items) of the reorder.
 template <bool MULTI>
-__global__ __launch_bounds__(kReorderBlock) void k_count_reorder(
-    const EngineDesc *__restrict__ ed, const SceneDesc *__restrict__ sd,
+__device__ __forceinline__ void reorder_blocks(
+    uint32_t b, uint32_t nblk, const EngineDesc *__restrict__ ed, const SceneDesc *__restrict__ sd,
     const uint32_t *__restrict__ scene_of, const uint32_t *__restrict__ keys,
     const uint32_t *__restrict__ bounds, const uint2 *__restrict__ order,
     uint32_t *__restrict__ sids, const float4 *__restrict__ vel, const float4 *__restrict__ pos,
@@ -577,7 +605,7 @@ __global__ __launch_bounds__(kReorderBlock) void k_count_reorder(
     __shared__ uint32_t next_key;
     const uint32_t n = ed->n_total, B = blockDim.x, tid = threadIdx.x;
     // whole workgroups iterate together (the cooperative rank synchronises)
-    for (uint32_t t0 = blockIdx.x * B; t0 < n; t0 += gridDim.x * B) {
+    for (uint32_t t0 = b * B; t0 < n; t0 += nblk * B) {
         const uint32_t t = t0 + tid;
         uint2 o = make_uint2(n, 0u);  // (resident slot, particle index)
         if (t < n) o = order[t];
@@ -648,8 +676,22 @@ __global__ __launch_bounds__(kReorderBlock) void k_count_reorder(
         if (rank) rank[o.y] = slot;  // only the Morton work order reads it
     }
     const uint32_t nc = ed->table_total + ed->n_scenes;
-    for (uint32_t i = global_id(); i < nc; i += grid_stride()) count[i] = 0u;
-    for (uint32_t i = global_id(); i < chain_words; i += grid_stride()) chain[i] = 0u;
+    for (uint32_t i = b * B + tid; i < nc; i += nblk * B) count[i] = 0u;
+    for (uint32_t i = b * B + tid; i < chain_words; i += nblk * B) chain[i] = 0u;
+}
+
+template <bool MULTI>
+__global__ __launch_bounds__(kReorderBlock) void k_count_reorder(
+    const EngineDesc *__restrict__ ed, const SceneDesc *__restrict__ sd,
+    const uint32_t *__restrict__ scene_of, const uint32_t *__restrict__ keys,
+    const uint32_t *__restrict__ bounds, const uint2 *__restrict__ order,
+    uint32_t *__restrict__ sids, const float4 *__restrict__ vel, const float4 *__restrict__ pos,
+    float4 *__restrict__ xps, float4 *__restrict__ vs, float4 *__restrict__ ps,
+    uint32_t *__restrict__ rank, uint32_t *__restrict__ count, uint64_t *__restrict__ chain,
+    uint32_t chain_words)
+{
+    reorder_blocks<MULTI>(blockIdx.x, gridDim.x, ed, sd, scene_of, keys, bounds, order, sids, vel, pos, xps,
+                          vs, ps, rank, count, chain, chain_words);
 }
 
 // Resident slots [0, n) of one scene into particle order:
@@ -1351,11 +1393,11 @@ constexpr uint32_t kListMax = 56;
 // block), and one XCD per slab would leave the others idle while it ends.
 constexpr uint32_t kXcdChunk = 128;
 template <bool WORK, bool GRID, bool MULTI = false>
-__device__ __forceinline__ uint32_t gather_index(uint32_t n)
+__device__ __forceinline__ uint32_t gather_index(uint32_t n, uint32_t b)
 {
-    if (!WORK && !GRID) return blockIdx.x * blockDim.x + threadIdx.x;
+    if (!WORK && !GRID) return b * blockDim.x + threadIdx.x;
     const uint32_t live = (n + blockDim.x - 1) / blockDim.x;
-    const uint32_t per = (live + 7u) >> 3, b = blockIdx.x, x = b & 7u, j = b >> 3;
+    const uint32_t per = (live + 7u) >> 3, x = b & 7u, j = b >> 3;
     uint32_t lb;
     if (j >= per) {
         lb = live + j;
@@ -1442,7 +1484,7 @@ __host__ __device__ constexpr uint32_t gather_wave_words(bool grid, bool nbr, bo
 
 #define SPH_GATHER_PROLOGUE()                                                       \
     extern __shared__ uint32_t lds_list[];                                          \
-    const uint32_t i = gather_index<WORK, GRID, MULTI>(ed->n_total);               \
+    const uint32_t i = gather_index<WORK, GRID, MULTI>(ed->n_total, vblock_);      \
     if (i >= ed->n_total) return;                                                   \
     const uint32_t g = WORK ? work[i] : svals[i];                                   \
     const uint32_t k = WORK ? rank[g] : i;                                          \
@@ -1473,6 +1515,8 @@ __host__ __device__ constexpr uint32_t gather_wave_words(bool grid, bool nbr, bo
     }                                                                               \
     ;                                                                               \
     run_scene<MULTI>(sd, scene_of, k, scene_body_)
+
+#define SPH_GATHER_FWD ed, sd, scene_of, work, rank, svals, xps, start, endt, nbr, ncnt, sflags
 
 #define SPH_GATHER_PARAMS                                                                     \
     const EngineDesc *__restrict__ ed, const SceneDesc *__restrict__ sd,                      \
@@ -1526,12 +1570,9 @@ __device__ __forceinline__ void gather_row(const SceneDesc &S, const float4 *__r
     }
 }
 
-// The second __launch_bounds__ argument is the minimum waves per SIMD the
-// register allocation must allow: the single-scene density pass fits 64
-// VGPRs (8 waves) and the viscosity pass 96 (5 waves) without spilling; the
-// multi-scene forms keep the occupancy they reach unconstrained.
+// Density of the slots of (virtual) workgroup vblock_ of the launch.
 template <bool MULTI, bool WORK, bool LIST, bool NBR, bool GRID>
-__global__ __launch_bounds__(256, MULTI ? 6 : 8) void k_density(SPH_GATHER_PARAMS, float4 *__restrict__ dens)
+__device__ __forceinline__ void density_vblock(uint32_t vblock_, SPH_GATHER_PARAMS, float4 *__restrict__ dens)
 {
     constexpr bool CONSUMER = false;
     SPH_GATHER_PROLOGUE();
@@ -1582,6 +1623,16 @@ __global__ __launch_bounds__(256, MULTI ? 6 : 8) void k_density(SPH_GATHER_PARAM
     SPH_GATHER_EPILOGUE();
 }
 
+// The second __launch_bounds__ argument is the minimum waves per SIMD the
+// register allocation must allow: the single-scene density pass fits 64
+// VGPRs (8 waves) and the viscosity pass 96 (5 waves) without spilling; the
+// multi-scene forms keep the occupancy they reach unconstrained.
+template <bool MULTI, bool WORK, bool LIST, bool NBR, bool GRID>
+__global__ __launch_bounds__(256, MULTI ? 6 : 8) void k_density(SPH_GATHER_PARAMS, float4 *__restrict__ dens)
+{
+    density_vblock<MULTI, WORK, LIST, NBR, GRID>(blockIdx.x, SPH_GATHER_FWD, dens);
+}
+
 // Density pass of the neighbour-row mode.  Each work-item stages its row in
 // LDS (row-major with a 65-word pitch, so the 64 lanes writing entry c of
 // their rows hit 32 distinct banks); after a barrier the workgroup writes its
@@ -1594,7 +1645,7 @@ __global__ __launch_bounds__(256) void k_density_rows(SPH_GATHER_PARAMS, float4 
 {
     extern __shared__ uint32_t lds_rows[];  // [blockDim.x][kRowPitch], then counts[blockDim.x]
     const uint32_t B = blockDim.x, t = threadIdx.x;
-    const uint32_t i = gather_index<true, false>(ed->n_total);
+    const uint32_t i = gather_index<true, false>(ed->n_total, blockIdx.x);
     const uint32_t n = ed->n_total;
     uint32_t *my = lds_rows + t * kRowPitch;
     uint32_t *cnt = lds_rows + B * kRowPitch;
@@ -1632,9 +1683,9 @@ __global__ __launch_bounds__(256) void k_density_rows(SPH_GATHER_PARAMS, float4 
 }
 
 template <bool MULTI, bool WORK, bool LIST, bool NBR, bool GRID>
-__global__ __launch_bounds__(256) void k_pressure(SPH_GATHER_PARAMS, const float4 *__restrict__ dens,
-                                                   const float4 *__restrict__ vs,
-                                                   float4 *__restrict__ vs2)
+__device__ __forceinline__ void pressure_vblock(uint32_t vblock_, SPH_GATHER_PARAMS,
+                                                const float4 *__restrict__ dens,
+                                                const float4 *__restrict__ vs, float4 *__restrict__ vs2)
 {
     constexpr bool CONSUMER = NBR && GRID;  // reads the neighbour masks
     SPH_GATHER_PROLOGUE();
@@ -1711,23 +1762,30 @@ __global__ __launch_bounds__(256) void k_pressure(SPH_GATHER_PARAMS, const float
     SPH_GATHER_EPILOGUE();
 }
 
+// (5 waves per SIMD: the single-scene form fits in 83 VGPRs; the multi-scene
+// form keeps 96 with two 8-byte spills, stored once at entry and reloaded
+// once at the end, instead of dropping to 4 waves)
+template <bool MULTI, bool WORK, bool LIST, bool NBR, bool GRID>
+__global__ __launch_bounds__(256, 5) void k_pressure(SPH_GATHER_PARAMS, const float4 *__restrict__ dens,
+                                                   const float4 *__restrict__ vs,
+                                                   float4 *__restrict__ vs2)
+{
+    pressure_vblock<MULTI, WORK, LIST, NBR, GRID>(blockIdx.x, SPH_GATHER_FWD, dens, vs, vs2);
+}
+
 // Writes the new state at the sorted slot (pos, vel, ids: the next resident
 // order) and records each particle's bucket for the next update's sort, with
 // this update's dt (record_bucket: keys, ticks, count, wkeys); rpos and
 // colors (may be null) are the render buffers, in particle order.
+#define SPH_VISCOSITY_PARAMS                                                                  \
+    const float4 *__restrict__ vs2, const float4 *__restrict__ ps, float4 *__restrict__ pos,         \
+        float4 *__restrict__ vel, uint32_t *__restrict__ ids, float4 *__restrict__ rpos,            \
+        float4 *__restrict__ colors, uint32_t *__restrict__ keys, uint32_t *__restrict__ ticks,     \
+        uint32_t *__restrict__ count, uint32_t *__restrict__ wkeys
+#define SPH_VISCOSITY_FWD vs2, ps, pos, vel, ids, rpos, colors, keys, ticks, count, wkeys
+
 template <bool MULTI, bool WORK, bool LIST, bool NBR, bool GRID>
-__global__ __launch_bounds__(256, MULTI ? 4 : 5) void k_viscosity_integrate(SPH_GATHER_PARAMS,
-                                                              const float4 *__restrict__ vs2,
-                                                              const float4 *__restrict__ ps,
-                                                              float4 *__restrict__ pos,
-                                                              float4 *__restrict__ vel,
-                                                              uint32_t *__restrict__ ids,
-                                                              float4 *__restrict__ rpos,
-                                                              float4 *__restrict__ colors,
-                                                              uint32_t *__restrict__ keys,
-                                                              uint32_t *__restrict__ ticks,
-                                                              uint32_t *__restrict__ count,
-                                                              uint32_t *__restrict__ wkeys)
+__device__ __forceinline__ void viscosity_vblock(uint32_t vblock_, SPH_GATHER_PARAMS, SPH_VISCOSITY_PARAMS)
 {
     constexpr bool CONSUMER = NBR && GRID;  // reads the neighbour masks
     SPH_GATHER_PROLOGUE();
@@ -1782,6 +1840,123 @@ __global__ __launch_bounds__(256, MULTI ? 4 : 5) void k_viscosity_integrate(SPH_
     record_bucket(*ed, S, predict_particle(S, x, v), true, k, keys, ticks, count, wkeys);
     SPH_GATHER_EPILOGUE();
 }
+
+template <bool MULTI, bool WORK, bool LIST, bool NBR, bool GRID>
+__global__ __launch_bounds__(256, MULTI ? 4 : 5) void k_viscosity_integrate(SPH_GATHER_PARAMS,
+                                                                           SPH_VISCOSITY_PARAMS)
+{
+    viscosity_vblock<MULTI, WORK, LIST, NBR, GRID>(blockIdx.x, SPH_GATHER_FWD, SPH_VISCOSITY_FWD);
+}
+
+// ---------------------------------------------------------------------------
+// Fused steady update: one persistent launch per Update
+// ---------------------------------------------------------------------------
+// The launch holds s.fused_grid workgroups of 256 -- no more than the device
+// keeps resident at once (occupancy x compute units, a multiple of 8) -- and
+// runs the six stages of a steady update (buckets recorded by the previous
+// update's viscosity pass) in order, with a grid barrier between stages: the
+// count scan by tickets (a workgroup waits only on chunks taken before its
+// own), scatter and reorder over the particles, then the three gathers over
+// the same virtual workgroups the separate launches use (virtual workgroup
+// v runs on workgroup v mod grid, so on the XCD the numbering assumes).
+// Each stage is the same device code as its own kernel, so results are
+// identical.  Barrier waits are bounded (kFusedBarrierSpin polls 64 x 64
+// clocks apart, about 0.1 s): a wait that reaches the bound raises scan_err
+// (the host then reports the update invalid) and the launch still ends.
+
+constexpr uint32_t kFusedBlock = 256;
+constexpr uint32_t kFusedBarrierSpin = 1u << 16;  // polls
+
+// Every workgroup arrives, in two levels so that no counter takes more than
+// 64 arrivals: workgroup b on sub-counter b mod 64 (each on a cache line of
+// its own), the last member of each sub-counter on the top counter; the last
+// arrival there resets it and advances the generation, and every other
+// workgroup waits for the generation to change (polling every 64 x 64
+// clocks).  Each counter is reset by its last arrival before the generation
+// moves, so all are zero at the next barrier and after the launch.
+// Agent-scope release and acquire publish each workgroup's stores to the
+// other XCDs' L2s.  bar: [0] top count, [1] generation, [64 + 16 q]
+// sub-count q (kFusedBarWords words).
+__device__ __forceinline__ void grid_barrier(uint32_t *bar, uint32_t nwg, uint32_t *err)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t gen = __hip_atomic_load(bar + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t q = blockIdx.x & 63u;
+        const uint32_t members = (nwg >> 6) + (q < (nwg & 63u) ? 1u : 0u);
+        const uint32_t subs = nwg < 64u ? nwg : 64u;
+        uint32_t *sub = bar + 64 + 16 * q;
+        bool last = false;
+        if (__hip_atomic_fetch_add(sub, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u == members) {
+            __hip_atomic_store(sub, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            last = __hip_atomic_fetch_add(bar, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u == subs;
+        }
+        if (last) {
+            __hip_atomic_store(bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(bar + 1, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            uint32_t spins = 0;
+            while (__hip_atomic_load(bar + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                if (++spins > kFusedBarrierSpin) {
+                    *err = 1u;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(64);
+            }
+        }
+    }
+    __syncthreads();
+}
+static_assert(64 + 16 * 63 < kFusedBarWords, "grid barrier words");
+
+template <bool MULTI>
+__global__ __launch_bounds__(kFusedBlock) void k_fused_update(const DevState s)
+{
+    __shared__ uint32_t chunk_s;
+    const uint32_t nwg = gridDim.x, b = blockIdx.x, tid = threadIdx.x;
+    // count scan: chunks by ticket (the counter starts at zero; the reorder
+    // below clears it for the next update)
+    const uint32_t na = scan_chunks_live(s.ed, s.count_blocks);
+    for (;;) {
+        __syncthreads();  // the previous chunk's shared values are read
+        if (tid == 0) chunk_s = atomicAdd(reinterpret_cast<uint32_t *>(s.chain + s.count_blocks), 1u);
+        __syncthreads();
+        const uint32_t c = chunk_s;
+        if (c >= na) break;
+        scan_chunk(c, na, s.ed, s.count, s.chain, s.bounds, s.sflags, s.scan_err);
+    }
+    grid_barrier(s.fbar, nwg, s.scan_err);
+    scatter_range(b * kFusedBlock + tid, nwg * kFusedBlock, s.ed, s.keys[0], s.keys[1], s.ids, s.bounds,
+                  s.order);
+    grid_barrier(s.fbar, nwg, s.scan_err);
+    reorder_blocks<MULTI>(b, nwg, s.ed, s.sd, s.scene_of, s.keys[0], s.bounds, s.order, s.sids, s.vel, s.pos,
+                          s.xps, s.vs, s.ps, nullptr, s.count, s.chain, s.count_blocks + 1);
+    grid_barrier(s.fbar, nwg, s.scan_err);
+    // the gathers over the virtual workgroups of the capacity's grid
+    const uint32_t vblocks = (((s.capacity + kFusedBlock - 1) / kFusedBlock) + 7u) & ~7u;
+    const uint32_t *const endt = s.bounds + 1;
+    for (uint32_t v = b; v < vblocks; v += nwg)
+        density_vblock<MULTI, false, false, true, true>(v, s.ed, s.sd, s.scene_of, nullptr, nullptr, s.sids,
+                                                        s.xps, s.bounds, endt, s.nbr, s.ncnt, s.sflags, s.dens);
+    grid_barrier(s.fbar, nwg, s.scan_err);
+    for (uint32_t v = b; v < vblocks; v += nwg)
+        pressure_vblock<MULTI, false, false, true, true>(v, s.ed, s.sd, s.scene_of, nullptr, nullptr, s.sids,
+                                                         s.xps, s.bounds, endt, s.nbr, s.ncnt, s.sflags, s.dens,
+                                                         s.vs, s.vs2);
+    grid_barrier(s.fbar, nwg, s.scan_err);
+    for (uint32_t v = b; v < vblocks; v += nwg)
+        viscosity_vblock<MULTI, false, false, true, true>(v, s.ed, s.sd, s.scene_of, nullptr, nullptr, s.sids,
+                                                          s.xps, s.bounds, endt, s.nbr, s.ncnt, s.sflags, s.vs2,
+                                                          s.ps, s.pos, s.vel, s.ids, s.rpos, s.colors, s.keys[0],
+                                                          s.keys[1], s.count, nullptr);
+}
+
+// dynamic LDS of the fused update's workgroup: the density pass's per-wave
+// staging, stream words and mask table (the consumers' table is no larger)
+constexpr size_t kFusedLds = sizeof(uint32_t) * kMaskTabWords * (kFusedBlock / 64);
+static_assert(gather_wave_words(true, true, false) <= kMaskTabWords &&
+                  gather_wave_words(true, true, true) <= kMaskTabWords,
+              "the fused update's LDS holds every gather stage's");
 
 // ---------------------------------------------------------------------------
 // Download / parity helpers (not on the step path)
@@ -2070,6 +2245,27 @@ hipError_t launch_viscosity_integrate(const DevState &s, const LaunchCfg &c, hip
                       s.vs2, s.ps, s.pos, s.vel, s.ids, s.rpos, s.colors, s.keys[0], s.keys[1],
                       s.count, s.grid ? nullptr : s.wkeys[0]);
     SPH_LAUNCH_CHECK();
+}
+
+hipError_t launch_fused_update(const DevState &s, hipStream_t st)
+{
+    if (!s.grid || s.fused_grid == 0 || s.fused_grid % 8u != 0) return hipErrorInvalidConfiguration;
+    if (s.multi) k_fused_update<true><<<s.fused_grid, kFusedBlock, kFusedLds, st>>>(s);
+    else k_fused_update<false><<<s.fused_grid, kFusedBlock, kFusedLds, st>>>(s);
+    SPH_LAUNCH_CHECK();
+}
+
+uint32_t fused_update_grid(bool multi, int device)
+{
+    int per_cu = 0, cus = 0;
+    hipError_t e = multi ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fused_update<true>, kFusedBlock,
+                                                                         kFusedLds)
+                         : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fused_update<false>, kFusedBlock,
+                                                                         kFusedLds);
+    if (e != hipSuccess || per_cu <= 0) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0)
+        return 0;
+    return (uint32_t)(per_cu * cus) & ~7u;
 }
 
 hipError_t launch_unpermute_dens(const DevState &s, float2 *out, hipStream_t st)

@@ -13,6 +13,8 @@ namespace sph {
 // buckets per workgroup of the one-pass count scan: a chain status word and a
 // ticket (one atomic on a single counter) per chunk
 constexpr uint32_t kCountChunkBuckets = 4096;
+// words of the fused update's grid barrier (counters on cache lines of their own)
+constexpr uint32_t kFusedBarWords = 64 + 16 * 64;
 
 struct DevState {
     // engine descriptors
@@ -60,7 +62,9 @@ struct DevState {
                          // (copied from the count flags by the bucket sort)
     uint32_t *nbr;       // [capacity * 64] neighbour rows by work index (nbr mode)
     uint32_t *ncnt;      // [capacity] neighbour count per row
-    uint32_t *scan_err;  // [1] raised when the count scan's bounded wait gave up
+    uint32_t *scan_err;  // [1] raised when a bounded wait gave up (count scan, grid barrier)
+    uint32_t *fbar;      // [kFusedBarWords] grid barrier of the fused update (zero between launches)
+    uint32_t fused_grid; // workgroups of the fused update that the device holds resident at once
     uint32_t capacity;       // particle slots allocated
     uint32_t table_capacity; // bucket slots allocated
 };
@@ -81,6 +85,10 @@ struct LaunchCfg {
     // nbr mode: density (in the work order) stores neighbour rows; pressure
     // and viscosity read them when their `rows` is set (then in the work order)
     bool nbr = false;
+    // grid key, neighbour masks in all three gathers: a steady update (buckets
+    // recorded) is one persistent launch with grid barriers between its
+    // stages (launch_fused_update) instead of one launch per stage
+    bool fused = false;
     int block_stream = 256;  // threads per block of streaming kernels
     bool needs_work() const { return nbr || density.work || pressure.work || viscosity.work; }
 };
@@ -111,6 +119,14 @@ inline LaunchCfg default_launch_cfg(bool grid)
 // Stage A + B1 when the previous update did not record the buckets: clears
 // the counts, then predict, cell, hash, bucket key and ticket of every particle.
 hipError_t launch_predict(const DevState &s, const LaunchCfg &c, hipStream_t st);
+// A steady update (buckets recorded by the previous one) as one launch of
+// s.fused_grid persistent workgroups: count scan, scatter, reorder, density,
+// pressure, viscosity+integrate, separated by grid barriers.  Grid key with
+// neighbour masks only (c.fused; see LaunchCfg).
+hipError_t launch_fused_update(const DevState &s, hipStream_t st);
+// Workgroups of the fused update that fit on the device at once (its
+// occupancy x compute units, a multiple of 8), or 0 when none does.
+uint32_t fused_update_grid(bool multi, int device);
 // Stages B2 + B3: counting sort into (bucket, particle index) order, bucket
 // bounds, gather of xp, v into sorted order; clears the counts.
 hipError_t launch_bucket_sort(const DevState &s, const LaunchCfg &c, hipStream_t st);

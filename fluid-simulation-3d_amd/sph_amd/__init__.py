@@ -273,15 +273,17 @@ def _variants_dict(v):
     d = {k: {"work": bool(v[4 * q]), "list": bool(v[4 * q + 1]), "block": int(v[4 * q + 2]),
              "rows": bool(v[4 * q + 3])} for q, k in enumerate(_KERNELS)}
     d["nbr"] = bool(v[12])
+    d["fused"] = bool(v[13])
     return d
 
 
 def _variants_array(d):
-    v = np.zeros(13, np.int32)
+    v = np.zeros(14, np.int32)
     for q, k in enumerate(_KERNELS):
         v[4 * q:4 * q + 4] = [int(d[k]["work"]), int(d[k]["list"]), int(d[k]["block"]),
                               int(d[k].get("rows", False))]
     v[12] = int(d.get("nbr", False))
+    v[13] = int(d.get("fused", False))
     return v
 
 
@@ -401,7 +403,7 @@ class Sim:
 
     @property
     def variants(self):
-        v = np.zeros(13, np.int32)
+        v = np.zeros(14, np.int32)
         _check(lib().sph_get_variants(self._h, _ptr(v)))
         return _variants_dict(v)
 
@@ -539,7 +541,7 @@ class Batch:
 
     @property
     def variants(self):
-        v = np.zeros(13, np.int32)
+        v = np.zeros(14, np.int32)
         _check(lib().sph_batch_get_variants(self._h, _ptr(v)))
         return _variants_dict(v)
 

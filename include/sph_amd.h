@@ -199,9 +199,15 @@ int sph_autotune(sph_handle h, float dt, int updates, double ms[3]);
  * v[4q+1] LDS list, v[4q+2] workgroup size (64, 128, 192, 256), v[4q+3] read
  * the row-mode records (pressure, viscosity); v[12] row mode: the density
  * pass records each particle's neighbours -- 64-entry neighbour rows under
- * the reference key, per-row 64-bit neighbour masks under the grid key. */
-int sph_get_variants(sph_handle h, int32_t v[13]);
-int sph_set_variants(sph_handle h, const int32_t v[13]);
+ * the reference key, the neighbour-mask record under the grid key; v[13]
+ * fused: under the grid key with the row mode in all three kernels, each
+ * steady update (buckets recorded by the previous one) is one persistent
+ * launch with grid barriers between its stages instead of one launch per
+ * stage (profiled updates keep the separate launches; a grid barrier's
+ * bounded wait that gives up is reported as SPH_ERR_DEVICE).  The array
+ * has 14 entries. */
+int sph_get_variants(sph_handle h, int32_t v[14]);
+int sph_set_variants(sph_handle h, const int32_t v[14]);
 /* Test support, no reference counterpart: the bound of the count scan's
  * look-back wait (the prefix sum behind the bucket table of UpdateSpatialLookup,
  * cc:486-496), in sleeps of 64 clocks; 65536 by default, about 1.7 ms.  A
@@ -318,8 +324,8 @@ int sph_batch_download(sph_batch b, uint32_t scene, float *pos4, float *vel4, fl
 int sph_batch_autotune(sph_batch b, float dt, int updates, double ms[3]);
 int sph_batch_profile_begin(sph_batch b, uint32_t max_updates);
 int sph_batch_profile_end(sph_batch b, double mean_ms[6], uint32_t *updates);
-int sph_batch_get_variants(sph_batch b, int32_t v[13]);
-int sph_batch_set_variants(sph_batch b, const int32_t v[13]);
+int sph_batch_get_variants(sph_batch b, int32_t v[14]);
+int sph_batch_set_variants(sph_batch b, const int32_t v[14]);
 
 /* ---- serving: continuous batching of bake requests ----
  * No reference counterpart.  A request is a scene (settings, particle count,
