@@ -163,3 +163,39 @@ def test_failed_growth_leaves_admission_undone():
         assert srv.stats()["queued"] == 0
     assert_bitexact(ra[0], _alone(*a, 2)[0], "a beside a failed admission")
     assert_bitexact(rb[1], _alone(*b, 2)[1], "b after waiting for room")
+
+
+def test_generated_requests_admitted_midrun_equal_alone():
+    # device-generated requests admitted at staggered ticks while others run
+    # and leave: every repack is device-only (moves, generation, the scene
+    # map filled on the device, descriptors through the staging ring, which
+    # wraps), with no stream synchronisation
+    specs = [(4096, 3, S.INIT_DAM_BREAK, 0), (6001, 2, S.INIT_CENTRED_BLOCK, 0),
+             (9000, 5, S.INIT_DAM_BREAK, 2), (4500, 2, S.INIT_CENTRED_BLOCK, 3),
+             (12000, 4, S.INIT_CENTRED_BLOCK, 5), (5003, 3, S.INIT_DAM_BREAK, 7),
+             (7777, 2, S.INIT_DAM_BREAK, 8), (20000, 3, S.INIT_CENTRED_BLOCK, 11),
+             (4099, 4, S.INIT_DAM_BREAK, 13), (8192, 2, S.INIT_CENTRED_BLOCK, 17)]
+    announce("generated mid-run", len(specs), [s[0] for s in specs])
+    plans = [W.serving_plan(n, kind, 500 + q) for q, (n, _, kind, _) in enumerate(specs)]
+    ids = [None] * len(specs)
+    with S.Server(hbm_budget_bytes=8 << 30, max_running=16, initial_capacity=1 << 17) as srv:
+        for tick in range(200):
+            for q, (n, steps, kind, at) in enumerate(specs):
+                if at == tick:
+                    p, gen = plans[q]
+                    ids[q] = srv.submit_generated(p, n, steps, kind, 500 + q, *gen)
+            if tick > max(s[3] for s in specs) and all(srv.status(i)[0] == S.REQ_DONE for i in ids):
+                break
+            srv.tick()
+        else:
+            raise AssertionError("requests did not finish")
+        got = [srv.fetch(i, n) for i, (n, _, _, _) in zip(ids, specs)]
+    for q, (n, steps, kind, _) in enumerate(specs):
+        p, gen = plans[q]
+        with S.Sim(p, n, S.TABLE_GRID) as sim:
+            sim.initialize(kind, 500 + q, *gen)
+            for _ in range(steps):
+                sim.step(W.DT, W.SUBSTEPS)
+            alone = sim.download()
+        for g, a, what in zip(got[q], alone, ("pos", "vel", "dens")):
+            assert_bitexact(g, a, f"request {q} (n={n}) {what}: served vs alone")
