@@ -20,7 +20,9 @@ Further legs (each measured here, on the same box, in the same run):
   dam_break_16k       16,384 particles, 200 steps (the CPU-reference config);
   serve               Poisson bake requests (20/s, 8,192-2,097,152 particles,
                       100-2,000 steps) by continuous batching, beside the
-                      plain path on the same arrivals, and a sustainable load.
+                      plain path on the same arrivals (handles' steps
+                      overlapped, and one request at a time), and a
+                      sustainable load.
 
 `roofline` is computed for the headline's dominant kernel from its mean
 duration, measured live over the timed region with HIP events recorded on the
@@ -363,11 +365,13 @@ def bench_serve(duration, rate, max_running, budget_gib, seed=4):
             "device_capacity_particles": st["capacity"]}
 
 
-def bench_serve_plain(duration, rate, max_running, budget_gib, seed=4):
+def bench_serve_plain(duration, rate, max_running, budget_gib, seed=4, overlap=True):
     """The plain one-scene-per-launch path on the same arrivals: each admitted
     request on its own handle (created at admission, state generated on the
-    device), every running handle's next step enqueued before any is waited
-    for, the same FIFO admission by slots and HBM budget."""
+    device), the same FIFO admission by slots and HBM budget.  overlap: every
+    running handle's next step enqueued before any is waited for (the
+    handles' streams overlap); else each request's step runs to completion
+    before the next request's is enqueued (one scene at a time)."""
     arrivals = W.serving_requests(duration, rate=rate, seed=seed)
     budget = budget_gib << 30
     queue, running = [], []  # running: [sim, steps_left, n, bytes]
@@ -395,9 +399,12 @@ def bench_serve_plain(duration, rate, max_running, budget_gib, seed=4):
             if nxt < len(arrivals):
                 time.sleep(max(0.0, min(0.01, arrivals[nxt][0] - (time.perf_counter() - t0))))
             continue
+        if overlap:
+            for r in running:
+                r[0].step_async(W.DT, W.SUBSTEPS)
         for r in running:
-            r[0].step_async(W.DT, W.SUBSTEPS)
-        for r in running:
+            if not overlap:
+                r[0].step_async(W.DT, W.SUBSTEPS)
             r[0].sync()
             r[1] -= 1
             psteps += r[2]
@@ -414,7 +421,7 @@ def bench_serve_plain(duration, rate, max_running, budget_gib, seed=4):
             "running_at_end": len(running), "queued_at_end": len(queue),
             "value": psteps / wall, "unit": "particle-steps/s", "wall_s": wall,
             "note": "each request on its own handle, one step of every running request enqueued "
-                    "per round"}
+                    "per round" + ("" if overlap else ", each waited for before the next is enqueued")}
 
 
 # ---------------------------------------------------------------------------
@@ -503,10 +510,15 @@ def run_serve(args):
     configured = bench_serve(args.serve_seconds, args.serve_rate, args.serve_slots, 200)
     plain = None if args.no_serve_plain else bench_serve_plain(args.serve_seconds, args.serve_rate,
                                                                 args.serve_slots, 200)
+    serial = None if args.no_serve_plain else bench_serve_plain(args.serve_seconds, args.serve_rate,
+                                                                 args.serve_slots, 200, overlap=False)
     return {
         "workload": "serving (Poisson bake requests, 8,192-2,097,152 particles, 100-2,000 steps)",
         "configured": configured, "plain_same_arrivals": plain,
+        "plain_one_at_a_time_same_arrivals": serial,
         "speedup_vs_plain": configured["value"] / plain["value"] if plain and plain["value"] else None,
+        "speedup_vs_one_at_a_time": configured["value"] / serial["value"] if serial and serial["value"]
+        else None,
         "sustained": None if args.serve_sustained_seconds <= 0 else
         bench_serve(args.serve_sustained_seconds, args.serve_sustained_rate, args.serve_slots, 200, seed=5)}
 

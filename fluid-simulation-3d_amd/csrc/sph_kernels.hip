@@ -1487,8 +1487,8 @@ __host__ __device__ constexpr uint32_t gather_wave_words(bool grid, bool nbr, bo
     const uint32_t i = gather_index<WORK, GRID, MULTI>(ed->n_total, vblock_);      \
     if (i >= ed->n_total) return;                                                   \
     const uint32_t g = WORK ? work[i] : svals[i];                                   \
-    const uint32_t k = WORK ? rank[g] : i;                                          \
-    const float4 pi = xps[k];                                                       \
+    const uint32_t k_slot_ = WORK ? rank[g] : i;                                    \
+    const float4 pi = xps[k_slot_];                                                 \
     uint32_t *lst = lds_list + threadIdx.x;                                         \
     /* GRID: each wave's LDS follows the lists (gather_wave_words words).    */  \
     /* The walking passes (CONSUMER false) stage rows there; with neighbour */  \
@@ -1503,6 +1503,11 @@ __host__ __device__ constexpr uint32_t gather_wave_words(bool grid, bool nbr, bo
     /* the rest of the kernel, for scene S (closed by SPH_GATHER_EPILOGUE,   */  \
     /* which runs it through run_scene)                                      */  \
     auto scene_body_ = [&](const SceneDesc &S) __attribute__((always_inline)) {    \
+    /* mask consumers: the slot, opaque to the optimiser at each pass of     */  \
+    /* run_scene's loop, so the record addresses derived from it are formed  */  \
+    /* where used, not hoisted out of the loop and kept live (spilled)       */  \
+    uint32_t k = k_slot_;                                                           \
+    if (CONSUMER) asm volatile("" : "+v"(k));                                       \
     /* grid key: no particle of this scene has its cell outside the grid's box */ \
     const bool clean = GRID && sflags[S.index] == 0u;                              \
     (void)mtab;                                                                     \
@@ -1514,7 +1519,7 @@ __host__ __device__ constexpr uint32_t gather_wave_words(bool grid, bool nbr, bo
 #define SPH_GATHER_EPILOGUE()                                                       \
     }                                                                               \
     ;                                                                               \
-    run_scene<MULTI>(sd, scene_of, k, scene_body_)
+    run_scene<MULTI>(sd, scene_of, k_slot_, scene_body_)
 
 #define SPH_GATHER_FWD ed, sd, scene_of, work, rank, svals, xps, start, endt, nbr, ncnt, sflags
 
@@ -1625,8 +1630,8 @@ __device__ __forceinline__ void density_vblock(uint32_t vblock_, SPH_GATHER_PARA
 
 // The second __launch_bounds__ argument is the minimum waves per SIMD the
 // register allocation must allow: the single-scene density pass fits 64
-// VGPRs (8 waves) and the viscosity pass 96 (5 waves) without spilling; the
-// multi-scene forms keep the occupancy they reach unconstrained.
+// VGPRs (8 waves); the multi-scene density pass keeps the 7 it reaches
+// unconstrained.
 template <bool MULTI, bool WORK, bool LIST, bool NBR, bool GRID>
 __global__ __launch_bounds__(256, MULTI ? 6 : 8) void k_density(SPH_GATHER_PARAMS, float4 *__restrict__ dens)
 {
@@ -1762,11 +1767,8 @@ __device__ __forceinline__ void pressure_vblock(uint32_t vblock_, SPH_GATHER_PAR
     SPH_GATHER_EPILOGUE();
 }
 
-// (5 waves per SIMD: the single-scene form fits in 83 VGPRs; the multi-scene
-// form keeps 96 with two 8-byte spills, stored once at entry and reloaded
-// once at the end, instead of dropping to 4 waves)
 template <bool MULTI, bool WORK, bool LIST, bool NBR, bool GRID>
-__global__ __launch_bounds__(256, 5) void k_pressure(SPH_GATHER_PARAMS, const float4 *__restrict__ dens,
+__global__ __launch_bounds__(256) void k_pressure(SPH_GATHER_PARAMS, const float4 *__restrict__ dens,
                                                    const float4 *__restrict__ vs,
                                                    float4 *__restrict__ vs2)
 {
@@ -1841,8 +1843,10 @@ __device__ __forceinline__ void viscosity_vblock(uint32_t vblock_, SPH_GATHER_PA
     SPH_GATHER_EPILOGUE();
 }
 
+// (5 waves per SIMD: 96 VGPRs, which the single-scene form fits in 76 and
+// the multi-scene form in exactly 96, without spilling)
 template <bool MULTI, bool WORK, bool LIST, bool NBR, bool GRID>
-__global__ __launch_bounds__(256, MULTI ? 4 : 5) void k_viscosity_integrate(SPH_GATHER_PARAMS,
+__global__ __launch_bounds__(256, 5) void k_viscosity_integrate(SPH_GATHER_PARAMS,
                                                                            SPH_VISCOSITY_PARAMS)
 {
     viscosity_vblock<MULTI, WORK, LIST, NBR, GRID>(blockIdx.x, SPH_GATHER_FWD, SPH_VISCOSITY_FWD);
