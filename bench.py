@@ -347,6 +347,7 @@ def bench_serve(duration, rate, max_running, budget_gib, seed=4):
             time.sleep(max(0.0, min(0.01, arrivals[nxt][0] - (time.perf_counter() - t0))))
     st = srv.stats()  # waits for the enqueued ticks
     wall = time.perf_counter() - t0
+    spread = tick_spread(*srv.tick_log())
     srv.close()
     psteps = st["particle_updates"] / 3.0
     return {"arrivals_per_s": rate, "window_s": duration, "submitted": submitted,
@@ -362,7 +363,48 @@ def bench_serve(duration, rate, max_running, budget_gib, seed=4):
                                    "p99": st["request_ms_p99"] if st["finished"] else None,
                                    "n": st["finished"]},
             "queue_wait_ms": {"p50": st["wait_ms_p50"], "p99": st["wait_ms_p99"]},
+            "step_latency_spread": spread,
             "device_capacity_particles": st["capacity"]}
+
+
+def tick_spread(ms, particles, layout):
+    """Where the spread of step latency comes from, from the server's tick
+    log.  within_membership: each tick over the median tick of its layout
+    (ticks between two membership changes advance the same requests), p50
+    and p99 of that ratio over layouts of 10 ticks or more.  tick_to_tick:
+    the factor between consecutive ticks of one layout (larger over smaller),
+    p50 and p99; it separates jitter from the slow change of a scene's cost as
+    it evolves.  by_load: p50/p99 step latency (3 ticks) of the ticks in each
+    range of particles advanced."""
+    if len(ms) == 0:
+        return None
+    ms = ms.astype(np.float64)
+    same = layout[1:] == layout[:-1]
+    step = np.maximum(ms[1:], ms[:-1])[same] / np.minimum(ms[1:], ms[:-1])[same]
+    rel = []
+    for q in np.unique(layout):
+        t = ms[layout == q]
+        if len(t) >= 10:
+            rel.append(t / np.median(t))
+    rel = np.concatenate(rel) if rel else np.zeros(0)
+    by_load = []
+    edges = [0, 1 << 18, 1 << 20, 1 << 22, 1 << 24, 1 << 62]
+    for lo, hi in zip(edges[:-1], edges[1:]):
+        t = ms[(particles >= lo) & (particles < hi)]
+        if len(t):
+            by_load.append({"particles": [lo, hi if hi < (1 << 62) else None], "ticks": int(len(t)),
+                            "step_ms_p50": 3 * float(np.percentile(t, 50)),
+                            "step_ms_p99": 3 * float(np.percentile(t, 99))})
+    return {"ticks": int(len(ms)), "layouts": int(len(np.unique(layout))),
+            "within_membership": None if not len(rel) else
+            {"ticks": int(len(rel)), "p50": float(np.percentile(rel, 50)),
+             "p99": float(np.percentile(rel, 99))},
+            "tick_to_tick": None if not len(step) else
+            {"pairs": int(len(step)), "p50": float(np.percentile(step, 50)),
+             "p99": float(np.percentile(step, 99))},
+            "particles_per_tick": {"p50": float(np.percentile(particles, 50)),
+                                   "p99": float(np.percentile(particles, 99))},
+            "by_load": by_load}
 
 
 def bench_serve_plain(duration, rate, max_running, budget_gib, seed=4, overlap=True):

@@ -83,7 +83,10 @@ struct sph_server_t {
     uint64_t next_id = 1;
     uint64_t used_bytes = 0;
     // statistics
-    std::vector<double> tick_ms;        // wall time of each tick with work
+    std::vector<double> tick_ms;        // GPU time of each tick with work
+    std::vector<uint64_t> tick_particles;  // particles it advanced
+    std::vector<uint32_t> tick_layout;     // layout_no it ran on
+    uint32_t layout_no = 0;                // repacks so far (membership changes)
     std::vector<double> request_ms;     // submit -> done of finished requests
     std::vector<double> wait_ms;        // submit -> admit
     uint64_t particle_updates = 0;      // particles x updates advanced
@@ -95,12 +98,23 @@ struct sph_server_t {
     static constexpr int kInFlight = 2;
     hipEvent_t ev_start[kRing] = {}, ev_end[kRing] = {};
     std::deque<int> pending;  // ring slots of enqueued, not yet harvested ticks
+    uint64_t slot_particles[kRing] = {};
+    uint32_t slot_layout[kRing] = {};
     uint64_t tick_no = 0;
 };
 
 namespace {
 
 int fail(int code, const std::string &msg) { return sph::set_last_error(code, msg); }
+
+// A completed tick's GPU time joins the statistics and the tick log.
+void record_tick(sph_server_t *s, int slot, float ms)
+{
+    s->tick_ms.push_back(ms);
+    s->tick_particles.push_back(s->slot_particles[slot]);
+    s->tick_layout.push_back(s->slot_layout[slot]);
+    s->busy_s += ms * 1e-3;
+}
 
 uint64_t request_bytes(uint32_t n, uint32_t table)
 {
@@ -201,6 +215,7 @@ int repack(sph_server_t *s, const std::vector<uint64_t> &admit, std::string *err
         r.vel.shrink_to_fit();
     }
     s->dirty = false;
+    ++s->layout_no;
     return SPH_OK;
 }
 
@@ -401,8 +416,7 @@ int sph_server_tick(sph_server s, sph_server_tick_info *info)
             float ms = 0.0f;
             if (hipEventElapsedTime(&ms, s->ev_start[slot], s->ev_end[slot]) != hipSuccess)
                 return fail(SPH_ERR_HIP, "tick timer");
-            s->tick_ms.push_back(ms);
-            s->busy_s += ms * 1e-3;
+            record_tick(s, slot, ms);
             s->pending.pop_front();
             wait = false;
         }
@@ -425,6 +439,8 @@ int sph_server_tick(sph_server s, sph_server_tick_info *info)
         }
         if (hipEventRecord(s->ev_end[slot], st) != hipSuccess) return fail(SPH_ERR_HIP, "tick event");
         s->pending.push_back(slot);
+        s->slot_layout[slot] = s->layout_no;
+        s->slot_particles[slot] = 0;  // set below, once the running requests are counted
         int rc = SPH_OK;
         // 3. retire finished requests: results to host, slot freed at the next repack
         for (size_t q = 0; q < s->running.size(); ++q) {
@@ -452,6 +468,7 @@ int sph_server_tick(sph_server s, sph_server_tick_info *info)
                 ++s->finished;
             }
         }
+        s->slot_particles[slot] = particles;
     }
     {
         int hr = harvest(false);
@@ -543,6 +560,23 @@ int sph_server_cancel(sph_server s, uint64_t id)
     SRV_GUARD_END
 }
 
+int sph_server_tick_log(sph_server s, uint64_t first, uint64_t cap, float *ms, uint64_t *particles,
+                        uint32_t *layout, uint64_t *count)
+{
+    SRV_GUARD_BEGIN
+    if (!s || !count) return fail(SPH_ERR_INVALID, "null argument");
+    const uint64_t have = s->tick_ms.size();
+    const uint64_t m = first < have ? std::min<uint64_t>(cap, have - first) : 0;
+    for (uint64_t i = 0; i < m; ++i) {
+        if (ms) ms[i] = (float)s->tick_ms[first + i];
+        if (particles) particles[i] = s->tick_particles[first + i];
+        if (layout) layout[i] = s->tick_layout[first + i];
+    }
+    *count = m;
+    return SPH_OK;
+    SRV_GUARD_END
+}
+
 int sph_server_get_stats(sph_server s, sph_server_stats *st)
 {
     SRV_GUARD_BEGIN
@@ -554,8 +588,7 @@ int sph_server_get_stats(sph_server s, sph_server_stats *st)
         if (hipEventSynchronize(s->ev_end[slot]) != hipSuccess ||
             hipEventElapsedTime(&ms, s->ev_start[slot], s->ev_end[slot]) != hipSuccess)
             return fail(SPH_ERR_HIP, "tick timer");
-        s->tick_ms.push_back(ms);
-        s->busy_s += ms * 1e-3;
+        record_tick(s, slot, ms);
         s->pending.pop_front();
     }
     memset(st, 0, sizeof(*st));

@@ -168,6 +168,7 @@ def lib():
         "sph_server_fetch": (i32, [H, u64, u32, p, p, p]),
         "sph_server_cancel": (i32, [H, u64]),
         "sph_server_get_stats": (i32, [H, ctypes.POINTER(ServerStats)]),
+        "sph_server_tick_log": (i32, [H, u64, u64, p, p, p, p]),
         "sph_batch_create": (i32, [u32, p, p, p, ctypes.POINTER(H)]),
         "sph_batch_destroy": (i32, [H]),
         "sph_batch_upload": (i32, [H, u32, p, p]),
@@ -631,3 +632,17 @@ class Server:
         st = ServerStats()
         _check(lib().sph_server_get_stats(self._h, ctypes.byref(st)))
         return _struct_dict(st)
+
+    def tick_log(self):
+        """(ms, particles, layout) arrays of every timed tick, in the order
+        they ran (stats() finishes timing the enqueued ones first)."""
+        cnt = ctypes.c_uint64(0)
+        total = self.stats()["ticks"]
+        ms = np.zeros(total, np.float32)
+        parts = np.zeros(total, np.uint64)
+        lay = np.zeros(total, np.uint32)
+        if total:
+            _check(lib().sph_server_tick_log(self._h, 0, total, ms.ctypes.data, parts.ctypes.data,
+                                             lay.ctypes.data, ctypes.byref(cnt)))
+        n = int(cnt.value)
+        return ms[:n], parts[:n], lay[:n]

@@ -400,6 +400,14 @@ int sph_server_status(sph_server s, uint64_t id, int32_t *state, uint64_t *updat
 int sph_server_fetch(sph_server s, uint64_t id, uint32_t n, float *pos4, float *vel4, float *dens2);
 int sph_server_cancel(sph_server s, uint64_t id);
 int sph_server_get_stats(sph_server s, sph_server_stats *st);
+/* The per-tick log behind the tick percentiles: for the timed ticks
+ * first .. first + cap - 1 (in the order they ran; sph_server_get_stats
+ * finishes timing the enqueued ones), each tick's GPU time, the particles it
+ * advanced and its layout number (the count of membership changes before it,
+ * so ticks of one number advanced the same requests).  Any output may be
+ * NULL; *count gets the number of entries written. */
+int sph_server_tick_log(sph_server s, uint64_t first, uint64_t cap, float *ms, uint64_t *particles,
+                        uint32_t *layout, uint64_t *count);
 
 /* ---- synthetic scenes: sph_generate writes host arrays; sph_initialize,
  * sph_batch_initialize and sph_server_submit_generated generate the same

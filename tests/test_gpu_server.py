@@ -199,3 +199,20 @@ def test_generated_requests_admitted_midrun_equal_alone():
             alone = sim.download()
         for g, a, what in zip(got[q], alone, ("pos", "vel", "dens")):
             assert_bitexact(g, a, f"request {q} (n={n}) {what}: served vs alone")
+
+
+def test_tick_log_counts_particles_and_membership():
+    a = _scene(5000, 31)
+    b = _scene(7000, 32)
+    announce("tick log", (5000, 7000))
+    with S.Server(max_running=4, initial_capacity=1 << 16) as srv:
+        ia = srv.submit(a[0], a[1], a[2], 2)    # 6 updates
+        ib = srv.submit(b[0], b[1], b[2], 4)    # 12 updates
+        _drain(srv, [ia, ib])
+        ms, parts, lay = srv.tick_log()
+        assert srv.stats()["ticks"] == len(ms) == 12
+    assert (ms > 0).all()
+    assert parts.tolist() == [12000] * 6 + [7000] * 6
+    # both admitted by one repack; the departure of the first is the next
+    assert len(set(lay[:6].tolist())) == 1 and len(set(lay[6:].tolist())) == 1
+    assert lay[6] == lay[0] + 1
