@@ -928,30 +928,33 @@ __device__ __forceinline__ uint64_t walk_bits(const float *sp, uint32_t len, con
 // (self included), as one bitstream: row r's run [rb_r, rb_r + L_r) gives L_r
 // bits, bit o set when slot rb_r + o passed, the rows in walk order, so row r
 // starts at bit off_r = L_0 + ... + L_{r-1}.  The neighbour-row buffer holds
-// bits[3][n] (uint4: the 384-bit stream; the third word only written, and
-// read, when the stream is longer than 256 bits) and meta[n] (uint2: the nine
-// lengths, 7 bits each, as a 64-bit word whose top bit, kMaskUnusable, marks
-// an unusable record).  A record is unusable -- the consumers walk that
-// particle instead -- when a row's three buckets are not consecutive (the z
-// wrap seam), a run exceeds 64 slots or the runs exceed 384 bits in all.
-// Otherwise a consumer re-derives each rb_r as the walk found it, from the
-// particle's cell (its position is unchanged between the passes) and the
+// bits[4][n] (uint4: the 512-bit stream; word w > 1 only written, and read,
+// when the stream is longer than 128 w bits) and then meta[n] (uint2: the
+// nine lengths, 7 bits each, as a 64-bit word whose top bit, kMaskUnusable,
+// marks an unusable record).  A record is unusable -- the consumers walk
+// that particle instead -- when a row's three buckets are not consecutive
+// (the z wrap seam), a run exceeds 64 slots or the runs exceed 512 bits in
+// all.  Otherwise a consumer re-derives each rb_r as the walk found it, from
+// the particle's cell (its position is unchanged between the passes) and the
 // bucket bounds (a table the wave's lanes share in cache), so the record is
-// 40 bytes (56 past 256 bits).  A lattice of spacing h / 2 gives 216 bits.
-constexpr uint32_t kMaskBitsMax = 384;
+// 40 bytes (56 past 256 bits, 72 past 384).  A lattice of spacing h / 2 gives
+// 216 bits; fluid compressed to twice that density (a dam break's impact)
+// about 350.
+constexpr uint32_t kMaskBitsMax = 512;
 constexpr uint32_t kMaskWords = kMaskBitsMax / 32;
+constexpr uint32_t kMaskMetaAt = kMaskWords / 2;  // meta[k] is uint2 [kMaskMetaAt n + k]
 constexpr uint64_t kMaskUnusable = 1ull << 63;
 
 struct MaskOut {
-    uint32_t *buf;    // neighbour-row buffer (>= 14 n words)
+    uint32_t *buf;    // neighbour-row buffer (>= 18 n words)
     uint32_t n;
     uint32_t *words;  // this lane's stream words in LDS (stride 64)
     uint64_t lens;    // out: the nine run lengths, 7 bits each
 };
 
 // stream |= mask << off on this lane's LDS words (stride 64).  The mask has
-// no bit at or past its run's length L and off + L <= 384, so every word
-// touched lies in the stream.
+// no bit at or past its run's length L and off + L <= kMaskBitsMax, so every
+// word touched lies in the stream.
 __device__ __forceinline__ void put_bits(uint32_t *mw, uint32_t off, uint64_t mask)
 {
     const uint32_t q = off >> 5, s = off & 31u;
@@ -1149,11 +1152,13 @@ __device__ __forceinline__ uint32_t walk_grid(const SceneDesc &S, const float4 *
     uint4 *bits = reinterpret_cast<uint4 *>(mo->buf);
     mo->lens = lens;
     const uint64_t meta = lens | (usable ? 0ull : kMaskUnusable);
-    reinterpret_cast<uint2 *>(mo->buf)[6 * (size_t)mo->n + k] = make_uint2((uint32_t)meta, (uint32_t)(meta >> 32));
+    reinterpret_cast<uint2 *>(mo->buf)[kMaskMetaAt * (size_t)mo->n + k] =
+        make_uint2((uint32_t)meta, (uint32_t)(meta >> 32));
     if (usable) {
         bits[k] = make_uint4(mw[0], mw[64], mw[128], mw[192]);
         bits[(size_t)mo->n + k] = make_uint4(mw[256], mw[320], mw[384], mw[448]);
         if (moff > 256u) bits[2 * (size_t)mo->n + k] = make_uint4(mw[512], mw[576], mw[640], mw[704]);
+        if (moff > 384u) bits[3 * (size_t)mo->n + k] = make_uint4(mw[768], mw[832], mw[896], mw[960]);
     }
     return usable ? 1u : 0u;
 }
@@ -1274,7 +1279,8 @@ __device__ __forceinline__ void gather_bits(const SceneDesc &S, const float4 *__
     for (int r = 0; r < 9; ++r) {
         const uint32_t L = (uint32_t)(lens >> (7 * r)) & 127u;
         // words past the stream's end only feed bits that the length mask
-        // drops (off + L <= 384, and off may be 384 when L = 0): clamp
+        // drops (off + L <= kMaskBitsMax, and off may be kMaskBitsMax when
+        // L = 0): clamp
         const uint32_t q = off >> 5, sh = off & 31u;
         const uint32_t w0 = mw[min(q, kMaskWords - 1u) * 64];
         const uint32_t w1 = mw[min(q + 1u, kMaskWords - 1u) * 64];
@@ -1333,6 +1339,13 @@ __device__ __forceinline__ void gather_record(const SceneDesc &S, const float4 *
         tab[576] = e.y;
         tab[640] = e.z;
         tab[704] = e.w;
+    }
+    if (total > 384u) {
+        const uint4 e = bits[3 * (size_t)n + k];
+        tab[768] = e.x;
+        tab[832] = e.y;
+        tab[896] = e.z;
+        tab[960] = e.w;
     }
     gather_bits<NB, SKIP_SELF, AUXN>(S, xps, aux, bounds, k, pi, lens, tab, tab, body);
 }
@@ -1748,7 +1761,7 @@ __device__ __forceinline__ void pressure_vblock(uint32_t vblock_, SPH_GATHER_PAR
         }
     };
     if (NBR && GRID) {
-        const uint2 mt2 = reinterpret_cast<const uint2 *>(nbr)[6 * (size_t)ed->n_total + k];
+        const uint2 mt2 = reinterpret_cast<const uint2 *>(nbr)[kMaskMetaAt * (size_t)ed->n_total + k];
         const uint64_t meta = (uint64_t)mt2.x | ((uint64_t)mt2.y << 32);
         if (!(meta & kMaskUnusable))
             gather_record<KB, true, 4>(S, xps, dens, start, nbr, ed->n_total, k, pi, meta, mtab, body);
@@ -1804,7 +1817,7 @@ __device__ __forceinline__ void viscosity_vblock(uint32_t vblock_, SPH_GATHER_PA
         fz += (vj.z - vi.z) * w;
     };
     if (NBR && GRID) {
-        const uint2 mt2 = reinterpret_cast<const uint2 *>(nbr)[6 * (size_t)ed->n_total + k];
+        const uint2 mt2 = reinterpret_cast<const uint2 *>(nbr)[kMaskMetaAt * (size_t)ed->n_total + k];
         const uint64_t meta = (uint64_t)mt2.x | ((uint64_t)mt2.y << 32);
         if (!(meta & kMaskUnusable))
             gather_record<4, true, 3>(S, xps, vs2, start, nbr, ed->n_total, k, pi, meta, mtab, body);

@@ -77,7 +77,7 @@ constexpr Layout make_layout()
     L.sids = take(4 * N);
     L.ncnt = take(4 * N);
     L.order = take(8 * N);
-    L.nbr = take(4 * 27 * N);  // neighbour masks: 9 rows x (64-bit mask, 32-bit base)
+    L.nbr = take(4 * 27 * N);  // neighbour-mask records: up to 18 words per particle used
     L.total = o;
     return L;
 }

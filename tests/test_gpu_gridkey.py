@@ -128,14 +128,39 @@ def _dense():
     return W.Scene("dense_8192", W.scene_params(), n, n, pos, vel, 1)
 
 
+def _lattice_08():
+    """Lattice spacing 0.08 (h / 2.5): row runs of 48-64 slots, streams of
+    about 300-500 bits, so a third of the records use the stream's fourth
+    128-bit word."""
+    n = 8192
+    pos, vel = S.generate(S.INIT_DAM_BREAK, 3, n, (4.0, 4.0, 4.0), 0.08, 0.001)
+    return W.Scene("lattice_08", W.scene_params(), n, n, pos, vel, 1)
+
+
+def _stream_over_512():
+    """2 x 2 x 5 particles in every cell (spacing 0.1, 0.1, 0.04 at fixed
+    offsets inside the cells): interior rows run 60 slots, within the 64-slot
+    limit, but nine of them make 540 bits, past the stream's 512, so those
+    records are unusable while the block's boundary layers keep usable ones."""
+    n = 16 * 16 * 32
+    pos, vel = S.generate(S.INIT_DAM_BREAK, 3, n, (4.0, 4.0, 4.0), 0.1, 0.0)
+    i, j, k = np.meshgrid(np.arange(16), np.arange(16), np.arange(32), indexing="ij")
+    pos[:, 0] = (-2.0 + 0.05 + 0.1 * i).reshape(-1)
+    pos[:, 1] = (-2.0 + 0.05 + 0.1 * j).reshape(-1)
+    pos[:, 2] = (-2.0 + 0.02 + 0.04 * k).reshape(-1)
+    return W.Scene("stream_over_512", W.scene_params(), n, n, pos, vel, 1)
+
+
 # (list, block, neighbour masks, pressure reads masks, viscosity reads masks)
 GRID_VARIANTS = [(l, b, 0, 0, 0) for l in (0, 1) for b in (64, 128, 256)] + \
     [(l, b, 1, 1, 1) for l in (0, 1) for b in (64, 256)] + [(0, 128, 1, 1, 0), (1, 128, 1, 0, 1)]
 
 
-@pytest.mark.parametrize("case", ["dam_break_16k", "outside_box_wrap", "dense"])
+@pytest.mark.parametrize("case", ["dam_break_16k", "outside_box_wrap", "dense", "lattice_08",
+                                  "stream_over_512"])
 def test_grid_key_every_variant_bitexact(case):
-    sc = _dense() if case == "dense" else _cases()[case]()
+    sc = {"dense": _dense, "lattice_08": _lattice_08,
+          "stream_over_512": _stream_over_512}.get(case, lambda: _cases()[case]())()
     dt = _sub_dt()
     ref = _oracle(sc, dt)
     with S.Sim(sc.params, sc.n, S.TABLE_GRID) as sim:
