@@ -566,16 +566,21 @@ def run_serve(args):
 
 
 def run_legs(names, args):
-    """The single-scene legs, each on its own handle."""
+    """The single-scene legs, each on its own handle, for the step counts of
+    their configurations (uniform_1m 1,000, two_blocks_16m 500, dam_break_16k
+    200) unless --scene-steps sets the first three."""
     legs = {}
     for name in names:
         if name == "uniform_1m_grid":
-            legs[name] = bench_scene(W.uniform_lower(), args.scene_steps, args.warmup)
+            sc = W.uniform_lower()
+            legs[name] = bench_scene(sc, args.scene_steps or sc.steps, args.warmup)
         elif name == "uniform_1m_reference_key":
-            legs[name] = bench_scene(W.uniform_lower(), 20, args.warmup, key="reference")
+            sc = W.uniform_lower()
+            legs[name] = bench_scene(sc, args.scene_steps or sc.steps, args.warmup, key="reference")
         elif name == "two_blocks_16m":
             c2 = W.two_blocks_plan()
-            legs[name] = bench_scene(c2["scene"], 10, 2, generated=c2["init"])
+            legs[name] = bench_scene(c2["scene"], args.scene_steps or c2["scene"].steps, 2,
+                                     generated=c2["init"])
         elif name == "dam_break_16k":
             c0 = W.dam_break(n=16384)
             legs[name] = bench_scene(c0, c0.steps, args.warmup, autotune=False)
@@ -591,7 +596,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the plain one-scene-per-launch paths (profiling runs)")
     ap.add_argument("--latency-steps", type=int, default=20)
-    ap.add_argument("--scene-steps", type=int, default=100, help="timed steps of uniform_1m")
+    ap.add_argument("--scene-steps", type=int, default=0,
+                    help="timed steps of the uniform_1m and two_blocks_16m legs (0: their "
+                         "configured 1,000 and 500)")
     ap.add_argument("--no-legs", action="store_true", help="headline only")
     ap.add_argument("--legs", default=",".join(LEGS),
                     help="comma-separated single-scene legs to run (default: all)")
