@@ -96,7 +96,8 @@ def kernel_name(stage, variants, grid, multi=False):
 
 
 # committed counter summaries are named for the workload's short name
-PROFILE_FILES = {"batch_256": "batch_256", "uniform_1048576": "uniform_1m"}
+PROFILE_FILES = {"batch_256": "batch_256", "uniform_1048576": "uniform_1m",
+                 "two_blocks_16777216": "two_blocks_16m"}
 
 
 def _profile(name):
