@@ -1248,10 +1248,14 @@ __device__ __forceinline__ void take_masks(const float4 *__restrict__ xps, const
 // stream first (row r: L_r bits from bit off_r), then the non-empty ones go
 // to this lane's LDS column of the mask table (tab: entry t's mask at words
 // 2t, 2t + 1 and its first slot at word 18 + t, stride 64), which may lie
-// over the stream words: they are all read by then.  The take loop's
-// next_row is then one table read.  With SKIP_SELF the particle's own slot
-// k is cleared from row 4 (dx = dy = 0), its own cell's run.
-constexpr uint32_t kMaskTabWords = 27 * 64;  // per wave
+// over the stream words: they are all read by then.  A ninth entry can only
+// be row 8 (entries are the non-empty rows in order), so its first slot stays
+// in a register (rb[8]) and the table is 26 words per lane: 6,656 bytes per
+// wave, six 256-work-item workgroups in a CU's 160 KiB of LDS (five at 27).
+// The take loop's next_row is then one table read.  With SKIP_SELF the
+// particle's own slot k is cleared from row 4 (dx = dy = 0), its own cell's
+// run.
+constexpr uint32_t kMaskTabWords = 26 * 64;  // per wave
 template <int NB, bool SKIP_SELF, int AUXN, typename Body>
 __device__ __forceinline__ void gather_bits(const SceneDesc &S, const float4 *__restrict__ xps,
                                             const float4 *__restrict__ aux,
@@ -1296,15 +1300,18 @@ __device__ __forceinline__ void gather_bits(const SceneDesc &S, const float4 *__
     for (int r = 0; r < 9; ++r) {
         tab[(2 * rows) * 64] = (uint32_t)rm[r];
         tab[(2 * rows + 1) * 64] = (uint32_t)(rm[r] >> 32);
-        tab[(18 + rows) * 64] = rb[r];
+        if (r < 8) tab[(18 + rows) * 64] = rb[r];  // rows <= r here
+        else if (rows < 8) tab[(18 + rows) * 64] = rb[8];
         rows += rm[r] != 0 ? 1u : 0u;
     }
+    const uint32_t rb8 = rb[8];
     uint32_t t = 0;  // next table entry
     take_masks<NB, AUXN>(xps, aux, k, pi,
                          [&](uint64_t &mask, uint32_t &base) {
                              if (t < rows) {
                                  mask = (uint64_t)tab[(2 * t) * 64] | ((uint64_t)tab[(2 * t + 1) * 64] << 32);
-                                 base = tab[(18 + t) * 64];
+                                 const uint32_t b = tab[(18 + min(t, 7u)) * 64];
+                                 base = t < 8u ? b : rb8;
                                  ++t;
                              }
                          },
