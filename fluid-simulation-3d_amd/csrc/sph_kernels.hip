@@ -1787,10 +1787,13 @@ __device__ __forceinline__ void pressure_vblock(uint32_t vblock_, SPH_GATHER_PAR
     SPH_GATHER_EPILOGUE();
 }
 
+// The single-scene mask consumer is held to 80 VGPRs (6 waves per SIMD,
+// what the LDS mask table allows; 82 unconstrained, 5 waves): its few spills
+// lie off the take loop (-0.4% to -1.2% in A/B runs).  The multi-scene one
+// keeps its 90 (at 80 its spills cost 5%).
 template <bool MULTI, bool WORK, bool LIST, bool NBR, bool GRID>
-__global__ __launch_bounds__(256) void k_pressure(SPH_GATHER_PARAMS, const float4 *__restrict__ dens,
-                                                   const float4 *__restrict__ vs,
-                                                   float4 *__restrict__ vs2)
+__global__ __launch_bounds__(256, (NBR && GRID && !MULTI) ? 6 : 1) void k_pressure(
+    SPH_GATHER_PARAMS, const float4 *__restrict__ dens, const float4 *__restrict__ vs, float4 *__restrict__ vs2)
 {
     pressure_vblock<MULTI, WORK, LIST, NBR, GRID>(blockIdx.x, SPH_GATHER_FWD, dens, vs, vs2);
 }
