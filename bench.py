@@ -184,6 +184,17 @@ def roofline(workload, n, variants, grid, per, multi):
             "valu": valu}
 
 
+def step_roofline(particle_steps_per_s, substeps, buckets_per_particle):
+    """The whole step against HBM with the reference's stage traffic per
+    particle-substep (each array read or written once per stage A-F,
+    neighbour re-reads on chip): 264 + 4 T/N bytes, T/N the bucket count per
+    particle (stage B1 clears the table)."""
+    b = 264.0 + 4.0 * buckets_per_particle
+    ach = b * particle_steps_per_s * substeps / 1e9
+    return {"alg_bytes_per_particle_substep": b, "achieved": ach, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": ach / HBM_PEAK_GBS}
+
+
 # ---------------------------------------------------------------------------
 # GPU legs
 # ---------------------------------------------------------------------------
@@ -317,6 +328,7 @@ def bench_scene(sc, steps, warmup, key="grid", latency_steps=20, autotune=True, 
            "stage_ms_per_update": per, "gather_variants": variants,
            "nonfinite_components": nonfinite}
     out["gather_kernels"] = gather_fractions(sc.name, sc.n, variants, grid, per, False)
+    out["step_roofline"] = step_roofline(out["value"], sc.substeps, buckets / sc.n)
     return out
 
 
@@ -669,6 +681,8 @@ def main():
             "speedup_vs_plain": batch["speedup_vs_plain"],
             "speedup_vs_one_at_a_time": batch["speedup_vs_one_at_a_time"]},
         "roofline": batch["roofline"],
+        # batch_256's scenes are configured with table = particle count
+        "step_roofline": step_roofline(batch["value"], W.SUBSTEPS, 1.0),
         "batch": batch,
     }
     if not args.no_legs:
