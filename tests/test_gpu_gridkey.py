@@ -153,11 +153,11 @@ def _stream_over_512():
 
 # (list, block, neighbour masks, pressure reads masks, viscosity reads masks)
 GRID_VARIANTS = [(l, b, 0, 0, 0) for l in (0, 1) for b in (64, 128, 256)] + \
-    [(l, b, 1, 1, 1) for l in (0, 1) for b in (64, 256)] + [(0, 128, 1, 1, 0), (1, 128, 1, 0, 1)]
+    [(l, b, 1, 1, 1) for l in (0, 1) for b in (64, 128, 256)] + [(0, 128, 1, 1, 0), (1, 128, 1, 0, 1)]
 
 
-@pytest.mark.parametrize("case", ["dam_break_16k", "outside_box_wrap", "dense", "lattice_08",
-                                  "stream_over_512"])
+@pytest.mark.parametrize("case", ["dam_break_16k", "uniform_64k", "outside_box_wrap", "dense",
+                                  "lattice_08", "stream_over_512"])
 def test_grid_key_every_variant_bitexact(case):
     sc = {"dense": _dense, "lattice_08": _lattice_08,
           "stream_over_512": _stream_over_512}.get(case, lambda: _cases()[case]())()
