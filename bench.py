@@ -392,7 +392,7 @@ def tick_spread(ms, particles, layout):
     if len(ms) == 0:
         return None
     ms = ms.astype(np.float64)
-    same = layout[1:] == layout[:-1]
+    same = (layout[1:] == layout[:-1]) & (ms[1:] > 0) & (ms[:-1] > 0)
     step = np.maximum(ms[1:], ms[:-1])[same] / np.minimum(ms[1:], ms[:-1])[same]
     rel = []
     for q in np.unique(layout):
