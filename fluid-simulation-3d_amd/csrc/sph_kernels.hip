@@ -1502,6 +1502,17 @@ __host__ __device__ constexpr uint32_t gather_wave_words(bool grid, bool nbr, bo
                                                                                        : kMaskTabWords);
 }
 
+// In a mask consumer, orders every lane's use of the wave's mask table
+// before the fallback walk that stages rows over it: the wave's lanes have
+// reconverged here, and the release/acquire pair at wavefront scope keeps the
+// table's LDS reads ahead of the staging copy's writes.
+#define SPH_TABLE_DONE()                                                            \
+    do {                                                                            \
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");                      \
+        __builtin_amdgcn_wave_barrier();                                            \
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");                      \
+    } while (0)
+
 #define SPH_GATHER_PROLOGUE()                                                       \
     extern __shared__ uint32_t lds_list[];                                          \
     const uint32_t i = gather_index<WORK, GRID, MULTI>(ed->n_total, vblock_);      \
@@ -1514,10 +1525,12 @@ __host__ __device__ constexpr uint32_t gather_wave_words(bool grid, bool nbr, bo
     /* The walking passes (CONSUMER false) stage rows there; with neighbour */  \
     /* masks the density pass's stream words (kMaskWords x 64) follow the   */  \
     /* staging, and its mask table lies over both.  The mask consumers hold */  \
-    /* only the table (their rare fallback walk reads global memory).       */  \
+    /* only the table; the particles whose record is unusable walk after   */  \
+    /* the wave's table users are done, staging rows over the table.       */  \
     uint32_t *wave_lds = lds_list + (LIST ? kListMax * blockDim.x : 0) +            \
                          (threadIdx.x >> 6) * gather_wave_words(GRID, NBR, CONSUMER); \
     float4 *stage = (GRID && !CONSUMER) ? reinterpret_cast<float4 *>(wave_lds) : nullptr; \
+    float4 *fallback_stage = GRID ? reinterpret_cast<float4 *>(wave_lds) : nullptr;    \
     uint32_t *mwords = wave_lds + 4 * kStageBuf + (threadIdx.x & 63u);            \
     uint32_t *mtab = wave_lds + (threadIdx.x & 63u);                               \
     /* the rest of the kernel, for scene S (closed by SPH_GATHER_EPILOGUE,   */  \
@@ -1531,6 +1544,7 @@ __host__ __device__ constexpr uint32_t gather_wave_words(bool grid, bool nbr, bo
     /* grid key: no particle of this scene has its cell outside the grid's box */ \
     const bool clean = GRID && sflags[S.index] == 0u;                              \
     (void)mtab;                                                                     \
+    (void)fallback_stage;                                                           \
     (void)mwords;                                                                   \
     (void)clean;                                                                    \
     (void)g;                                                                        \
@@ -1770,9 +1784,10 @@ __device__ __forceinline__ void pressure_vblock(uint32_t vblock_, SPH_GATHER_PAR
     if (NBR && GRID) {
         const uint2 mt2 = reinterpret_cast<const uint2 *>(nbr)[kMaskMetaAt * (size_t)ed->n_total + k];
         const uint64_t meta = (uint64_t)mt2.x | ((uint64_t)mt2.y << 32);
-        if (!(meta & kMaskUnusable))
-            gather_record<KB, true, 4>(S, xps, dens, start, nbr, ed->n_total, k, pi, meta, mtab, body);
-        else gather<GRID, false, true, true>(S, xps, dens, start, endt, lst, stage, clean, k, pi, body);
+        const bool usable = !(meta & kMaskUnusable);
+        if (usable) gather_record<KB, true, 4>(S, xps, dens, start, nbr, ed->n_total, k, pi, meta, mtab, body);
+        SPH_TABLE_DONE();
+        if (!usable) gather<GRID, false, true, true>(S, xps, dens, start, endt, lst, fallback_stage, clean, k, pi, body);
     } else if (NBR) {
         gather_row(S, xps, dens, start, endt, nbr + (size_t)i * kNbrMax, ncnt[i], k, pi, body);
     } else {
@@ -1829,9 +1844,10 @@ __device__ __forceinline__ void viscosity_vblock(uint32_t vblock_, SPH_GATHER_PA
     if (NBR && GRID) {
         const uint2 mt2 = reinterpret_cast<const uint2 *>(nbr)[kMaskMetaAt * (size_t)ed->n_total + k];
         const uint64_t meta = (uint64_t)mt2.x | ((uint64_t)mt2.y << 32);
-        if (!(meta & kMaskUnusable))
-            gather_record<4, true, 3>(S, xps, vs2, start, nbr, ed->n_total, k, pi, meta, mtab, body);
-        else gather<GRID, false, true, true>(S, xps, vs2, start, endt, lst, stage, clean, k, pi, body);
+        const bool usable = !(meta & kMaskUnusable);
+        if (usable) gather_record<4, true, 3>(S, xps, vs2, start, nbr, ed->n_total, k, pi, meta, mtab, body);
+        SPH_TABLE_DONE();
+        if (!usable) gather<GRID, false, true, true>(S, xps, vs2, start, endt, lst, fallback_stage, clean, k, pi, body);
     } else if (NBR) {
         gather_row(S, xps, vs2, start, endt, nbr + (size_t)i * kNbrMax, ncnt[i], k, pi, body);
     } else {
