@@ -69,3 +69,40 @@ def test_colliding_blocks_rollout_bitexact(blocks):
     assert_bitexact(dens_g, dens_o, "densities after the collision")
     # the case covers overflowing records (about 45% of the particles here)
     assert _overflow_share(pos_o, sc.params.smoothing_radius) > 0.2
+
+
+def test_colliding_blocks_in_a_batch_bitexact():
+    """The same collision as one scene of a multi-scene batch, beside a dam
+    break and a uniform scene: the multi-scene kernels take each wave's
+    scenes in turn, and the overflowing records' walks stage their rows
+    inside that loop."""
+    plan = W.two_blocks_plan(n=N, box=(6.0, 3.0, 6.0), speed=16.0)
+    kind, seed, box, spacing, jitter, speed = plan["init"]
+    pos, vel = S.generate(kind, seed, N, box, spacing, jitter, speed)
+    scenes = [(plan["scene"].params, pos, vel),
+              (lambda d: (d.params, d.pos, d.vel))(W.dam_break(n=16384)),
+              (lambda u: (u.params, u.pos, u.vel))(W.uniform_lower(n=32768, table=65536, box=(4.0, 2.0, 4.0)))]
+    steps = 10
+    announce("colliding blocks in a batch", [len(p) for _, p, _ in scenes], "steps", steps)
+    b = S.Batch([p for p, _, _ in scenes], [len(x) for _, x, _ in scenes])
+    try:
+        v = b.variants
+        for k, blk in zip(("density", "pressure", "viscosity"), (256, 256, 128)):
+            v[k] = {"work": False, "list": False, "block": blk, "rows": True}
+        v["nbr"] = True
+        b.variants = v
+        assert b.variants["nbr"] and b.variants["viscosity"]["rows"]
+        for q, (_, x, u) in enumerate(scenes):
+            b.upload(q, x, u)
+        for _ in range(steps):
+            b.step_async(W.DT, W.SUBSTEPS)
+        b.sync()
+        got = [b.download(q) for q in range(len(scenes))]
+    finally:
+        b.close()
+    O.set_threads(16)
+    for q, (p, x, u) in enumerate(scenes):
+        pos_o, vel_o, dens_o = O.update(orc_params(p), _sub_dt(), x, u, T=len(x), substeps=steps * W.SUBSTEPS)
+        assert_bitexact(got[q][0][:, :3], pos_o[:, :3], f"scene {q} positions")
+        assert_bitexact(got[q][1][:, :3], vel_o[:, :3], f"scene {q} velocities")
+        assert_bitexact(got[q][2], dens_o, f"scene {q} densities")
