@@ -41,7 +41,7 @@ def _overflow_share(pos, h):
     return float(((longest > 64) | (tot > 512)).mean())
 
 
-@pytest.mark.parametrize("blocks", [None, (128, 64, 256)])
+@pytest.mark.parametrize("blocks", [None, (128, 64, 256), "fused"])
 def test_colliding_blocks_rollout_bitexact(blocks):
     plan = W.two_blocks_plan(n=N, box=(6.0, 3.0, 6.0), speed=16.0)
     sc = plan["scene"]
@@ -51,10 +51,13 @@ def test_colliding_blocks_rollout_bitexact(blocks):
     with S.Sim(sc.params, N, S.TABLE_GRID) as sim:
         if blocks is not None:
             v = sim.variants
-            for k, b in zip(("density", "pressure", "viscosity"), blocks):
+            fused = blocks == "fused"  # one persistent launch per steady Update
+            for k, b in zip(("density", "pressure", "viscosity"), (256, 256, 256) if fused else blocks):
                 v[k] = {"work": False, "list": False, "block": b, "rows": True}
             v["nbr"] = True
+            v["fused"] = fused
             sim.variants = v
+            assert sim.variants["fused"] == fused
         assert sim.variants["nbr"] and sim.variants["pressure"]["rows"]
         sim.upload(pos, vel)
         for _ in range(STEPS):
