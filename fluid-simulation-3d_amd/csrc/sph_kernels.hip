@@ -819,9 +819,12 @@ __device__ __forceinline__ void walk_hash(const SceneDesc &S, const float4 *__re
 // nearby records then hit distinct banks, where 16-byte records at a 16-byte
 // stride made lanes eight records apart collide.
 constexpr int KBG = 4;                          // records per batch of the grid walk
-constexpr uint32_t kStageCap = 120;             // staged records per wave and row
-// floats per plane: the cap + padding (a batch may read KBG-1 past the end)
-constexpr uint32_t kStageBuf = 144;
+constexpr uint32_t kStageCap = 156;             // staged records per wave and row
+// floats per plane: the cap + padding (a batch may read KBG-1 past the end).
+// Four planes and the density pass's 512-bit stream words (16 per lane) fill
+// the 26 words per lane of the mask table exactly, so the cap costs no LDS;
+// compressed fluid (16-30 particles a cell) needs ranges past 120 records.
+constexpr uint32_t kStageBuf = 160;
 static_assert(kStageBuf >= kStageCap + KBG, "staging planes");
 
 // Slots [b, e) of a grid row: the buckets of the row's cells dz = -1, 0, 1
