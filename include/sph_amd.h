@@ -405,7 +405,9 @@ int sph_server_get_stats(sph_server s, sph_server_stats *st);
  * finishes timing the enqueued ones), each tick's GPU time, the particles it
  * advanced and its layout number (the count of membership changes before it,
  * so ticks of one number advanced the same requests).  Any output may be
- * NULL; *count gets the number of entries written. */
+ * NULL; *count gets the number of entries written.  The server keeps every
+ * tick since its creation (28 bytes each, with the tick percentiles'
+ * samples) until it is destroyed. */
 int sph_server_tick_log(sph_server s, uint64_t first, uint64_t cap, float *ms, uint64_t *particles,
                         uint32_t *layout, uint64_t *count);
 
