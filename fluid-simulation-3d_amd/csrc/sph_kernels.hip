@@ -1088,8 +1088,11 @@ __device__ __forceinline__ uint32_t walk_grid(const SceneDesc &S, const float4 *
             auto inline_body = [&](uint32_t m, float ox, float oy, float oz, float d2, const float4 &a) {
                 if (!MASK_ONLY && !defer) body(m, ox, oy, oz, d2, a);
             };
+            // MASK_ONLY: a lane whose run passes 64 slots gets the first 64
+            // bits (its reads stay inside its staged run) and its record is
+            // unusable below, so the wave keeps the branch-free walk
             if ((DEFER || MASK_ONLY) && !SKIP_SELF && !AUX && staged && nohash &&
-                __ballot(re - rb <= 64u) == __ballot(1))
+                (MASK_ONLY || __ballot(re - rb <= 64u) == __ballot(1)))
                 mask = walk_bits(reinterpret_cast<const float *>(stage) + (rb - lo), re - rb, pi, thr);
             else if (staged && nohash)
                 walk_run<true, SKIP_SELF, AUX, MASKS || DEFER, false>(stage, lo, xps, aux, k, pi, thr, rb, re,
