@@ -1508,10 +1508,10 @@ __host__ __device__ constexpr uint32_t gather_wave_words(bool grid, bool nbr, bo
                                                                                        : kMaskTabWords);
 }
 
-// In a mask consumer, orders every lane's use of the wave's mask table
-// before the fallback walk that stages rows over it: the wave's lanes have
-// reconverged here, and the release/acquire pair at wavefront scope keeps the
-// table's LDS reads ahead of the staging copy's writes.
+// Orders every lane's use of the wave's mask table before the walk of the
+// particles whose record is unusable, which stages rows over the table: the
+// wave's lanes have reconverged here, and the release/acquire pair at
+// wavefront scope keeps the table's LDS reads ahead of the staging writes.
 #define SPH_TABLE_DONE()                                                            \
     do {                                                                            \
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");                      \
@@ -1641,8 +1641,8 @@ __device__ __forceinline__ void density_vblock(uint32_t vblock_, SPH_GATHER_PARA
     // LDS column, the record in the neighbour-row buffer) and the bodies then
     // run over all of them, decoded from LDS into the mask table, which lies
     // over the wave's staging planes (free once every lane's walk is done);
-    // a particle whose record is unusable walks again with the body, from
-    // global memory (other lanes of the wave hold their tables there).
+    // a particle whose record is unusable walks again with the body once the
+    // wave's other lanes are done with their tables, staging its rows there.
     // Grid key without masks or the LDS list: the body runs over each row's
     // mask as the walk goes (DEFER).
     MaskOut mo{nbr, ed->n_total, mwords, 0};
@@ -1653,9 +1653,9 @@ __device__ __forceinline__ void density_vblock(uint32_t vblock_, SPH_GATHER_PARA
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if (ok)
-            gather_bits<4, false, 0>(S, xps, nullptr, start, k, pi, mo.lens, mwords, mtab, body);
-        else walk<GRID, false, false>(S, xps, nullptr, start, endt, k, pi, nullptr, clean, body);
+        if (ok) gather_bits<4, false, 0>(S, xps, nullptr, start, k, pi, mo.lens, mwords, mtab, body);
+        SPH_TABLE_DONE();
+        if (!ok) walk<GRID, false, false, false, true>(S, xps, nullptr, start, endt, k, pi, stage, clean, body);
     } else if (GRID && !LIST)
         walk<GRID, false, false, false, true>(S, xps, nullptr, start, endt, k, pi, stage, clean, body);
     else
